@@ -1,0 +1,195 @@
+#!/usr/bin/env python
+"""Benchmark of the time-stepping hot path on one MI355X.
+
+Workload (default): the 1,038,230-node Velocity-Verlet beam -- 470 x 47 x 47
+jittered lattice (seed 1) in a 2.0 x 0.2 x 0.2 m bar, horizon 3.01 dx,
+109,555,662 directed bonds, rho 2400, dt 1e-8, a 1e4 N/m^3 body force on the
+central strip, supports at both ends.  A step is one launch of the fused
+step kernel: break test + bond-force gather over every live bond, force BC,
+Velocity-Verlet update.  Inputs are resident in HBM before timing.
+
+Prints one JSON line:
+  value          bond-updates/s = live directed bonds x steps / time
+  roofline       the step kernel against HBM: algorithmic bytes per launch
+                 (4 B per live bond + 276 B per node, the reference's array
+                 formats each touched once) / average launch time, measured
+                 with HIP events on the library's stream; peak 8 TB/s
+  cpu_baseline   the oracle (the reference's Cython kernels restated in C,
+                 one core) on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("FP64 bond-updates/s + %HBM peak at 1M nodes; batched vs plain "
+          "sample-steps/s")
+HBM_PEAK_GBS = 8000.0
+B_NODE = {"euler": 156, "euler_cromer": 252, "velocity_verlet": 276}
+
+
+def algorithmic_bytes(w, live):
+    """Bytes one step must move at least: nlist entries of live bonds (4 B)
+    plus the per-node arrays of the integrator, each touched once; n-linear
+    models add bond_types + regimes (8 B per live bond)."""
+    b = 4 * live + B_NODE[w.integrator] * w.nnodes
+    if w.nregimes > 1 or w.nbond_types > 1:
+        b += 8 * live
+    return b
+
+
+def cpu_baseline(w, budget_s):
+    """Time the oracle on this workload: break_bonds + bond_force
+    (peridynamics.pyx order, one thread) and the integrator's update, whole
+    steps until budget_s has passed (at least one)."""
+    import oracle
+    n = w.nnodes
+    coords = w.coords
+    nlist = np.array(w.nlist, dtype=np.int32)
+    n_neigh = np.array(w.n_neigh, dtype=np.int32)
+    u = np.zeros((n, 3))
+    ud = np.zeros((n, 3))
+    udd = np.zeros((n, 3))
+    L = oracle.lib()
+    d, f = w.magnitudes(10000)
+    kind = w.integrator
+    steps = 0
+    live = int(n_neigh.sum())
+    t0 = time.perf_counter()
+    while True:
+        oracle.break_bonds(coords + u, coords, nlist, n_neigh,
+                           w.critical_stretch)
+        force = oracle.bond_force(coords + u, coords, nlist, n_neigh,
+                                  w.volume, w.bond_stiffness,
+                                  w.force_bc_values, w.force_bc_types,
+                                  f[steps])
+        if kind == "euler":
+            oracle.update_displacement(u, w.bc_values, w.bc_types, force,
+                                       d[steps], w.dt)
+        else:
+            fn = (L.oracle_update_velocity_verlet
+                  if kind == "velocity_verlet"
+                  else L.oracle_update_euler_cromer)
+            fn(n, oracle._d(force), oracle._d(u), oracle._d(ud),
+               oracle._d(udd), oracle._i(w.bc_types),
+               oracle._d(w.bc_values), oracle._d(w.densities), d[steps],
+               w.damping, w.dt)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": live * steps / elapsed,
+        "unit": "bond-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{steps} full steps of the {n}-node workload "
+                   f"({elapsed:.1f} s): peridynamics.pyx break_bonds + "
+                   "bond_force restated in C (gcc -O3), plus the "
+                   f"{kind} update"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="beam_vv",
+                    choices=["beam_vv", "composite_ec", "bar_euler"])
+    ap.add_argument("--lanes-per-node", type=int, default=1)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU baseline budget (0 to skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
+                          "sharding is a design (see the design notes), "
+                          "not an implemented path", "n_gpus": args.gpus}))
+        return 2
+
+    from probabilistic_peridynamics_amd import _lib, configs
+
+    if _lib.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP GPU")
+    t_setup = time.perf_counter()
+    w = getattr(configs, args.workload)()
+    integ = configs.make_integrator(w)
+    integ.set_launch(args.lanes_per_node, args.block)
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] {w.name}: N={w.nnodes} L={w.live_bonds()} "
+          f"K={w.max_neighbours} setup {setup_s:.1f}s", file=sys.stderr,
+          flush=True)
+
+    dm, fm = w.magnitudes(args.warmup + args.steps)
+    integ.run(args.warmup, dm[:args.warmup], fm[:args.warmup])
+    integ.synchronize()
+    live0 = integ.live_bonds()
+    integ.synchronize()
+    t0 = time.perf_counter()
+    ms = integ.timed_run(args.steps, dm[args.warmup:], fm[args.warmup:])
+    integ.synchronize()
+    wall = time.perf_counter() - t0
+    live1 = integ.live_bonds()
+    live = 0.5 * (live0 + live1)
+    per_step_s = ms / 1e3 / args.steps
+    value = live / per_step_s
+    B = algorithmic_bytes(w, live)
+    achieved = B / per_step_s / 1e9
+    print(f"[bench] {args.steps} steps: {ms:.3f} ms (events), {wall*1e3:.3f}"
+          f" ms wall, live {live0}->{live1}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        cpu["threads_available"] = os.cpu_count()
+
+    name, arch = _lib.device_name(0)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "nodes": w.nnodes,
+            "live_bonds": live,
+            "max_neighbours": w.max_neighbours,
+            "integrator": w.integrator,
+            "lanes_per_node": args.lanes_per_node,
+            "block": args.block,
+            "device": f"{name} ({arch})",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": B,
+            "launch_us": per_step_s * 1e6,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

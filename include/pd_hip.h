@@ -1,0 +1,177 @@
+/*
+ * pd_hip.h -- C ABI of libpd_hip.so, the MI355X (gfx950) bond-based
+ * peridynamics time-stepping library.
+ *
+ * Every entry point takes plain pointers and sizes, copies what it reads
+ * before returning, and returns 0 on success or a negative PD_E_* code; the
+ * message of the last failure on the calling thread is pd_last_error().
+ * Host arrays are C-contiguous, float64 ("double") or int32 ("int"), in the
+ * reference's shapes: (nnodes, 3) for per-node vectors, (nnodes,
+ * max_neighbours) for per-bond arrays.
+ *
+ * The entry points mirror the reference's Integrator interface
+ * (peripy/integrators.py), which is what a binding of this path replaces:
+ *
+ *   pd_device_count      <- peripy/cl/utilities.py:28-44 get_context
+ *   pd_build             <- Integrator.build, integrators.py:84-188
+ *   pd_create_buffers    <- Integrator.create_buffers, integrators.py:190-254
+ *   pd_step              <- Integrator.__call__, integrators.py:330-349,
+ *                           484-503, 586-606, 700-720
+ *   pd_run               <- the step loop of Model.simulate, model.py:1295-1301
+ *   pd_write             <- Integrator.write, integrators.py:285-300
+ *   pd_destroy           <- (Python garbage collection of the integrator)
+ *
+ * Kernel semantics: see the per-function comments and
+ * probabilistic_peridynamics_amd/csrc/pd_hip.hip.
+ */
+#ifndef PD_HIP_H
+#define PD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pd_sim pd_sim;
+
+/* integrator kinds */
+#define PD_EULER 0            /* peridynamics.pyx:159-185, cl/euler.cl:3-24 */
+#define PD_EULER_CROMER 1     /* cl/euler_cromer.cl:3-35 */
+#define PD_VELOCITY_VERLET 2  /* cl/velocity_verlet.cl:3-36 */
+
+/* neighbour-list layouts (model.py:575-585 chooses by integrator.context) */
+#define PD_LAYOUT_COMPACT 0  /* Cython: width family.max(), broken bonds are
+                                removed by swap-with-last compaction
+                                (peridynamics.pyx:102-156) */
+#define PD_LAYOUT_SLOT 1     /* OpenCL: width a power of two, -1 marks a broken
+                                or padding slot (cl/peridynamics.cl:91) */
+
+/* error codes */
+#define PD_OK 0
+#define PD_E_ARG -1     /* invalid argument (shape, value, missing array) */
+#define PD_E_HIP -2     /* a HIP runtime call failed */
+#define PD_E_NODEV -3   /* no usable GPU */
+#define PD_E_STATE -4   /* call out of order (e.g. step before buffers) */
+#define PD_E_LIMIT -5   /* a size limit of this library was exceeded */
+
+/* Last error message of the calling thread ("" if none). */
+const char *pd_last_error(void);
+
+/* Number of visible HIP devices; 0 (and PD_OK) when none.  Replaces
+ * get_context (peripy/cl/utilities.py:28-44): the caller raises its
+ * ContextError when *count == 0. */
+int pd_device_count(int *count);
+
+/* Name and architecture of a device (e.g. "gfx950"), NUL-terminated. */
+int pd_device_name(int device, char *name, int name_len, char *arch,
+                   int arch_len);
+
+/* Integrator.build (integrators.py:84-188): static per-node arrays.
+ *   coords (N,3) f64, volume (N) f64, family (N) i32, bc_types (N,3) i32,
+ *   bc_values (N,3) f64, force_bc_types (N,3) i32, force_bc_values (N,3) f64,
+ *   stiffness_corrections (N,K) f64 or NULL (bond_force2/4),
+ *   bond_types (N,K) i32 or NULL (bond_force3/4),
+ *   densities (N,3) f64: required for EULER_CROMER / VELOCITY_VERLET and
+ *   rejected for EULER (integrators.py:415-423, 511-519, 610-620).
+ * max_neighbours is K, the width of every (N,K) array. */
+int pd_build(pd_sim **sim, int device, int integrator, int layout, double dt,
+             double damping, int nnodes, int max_neighbours,
+             const double *coords, const double *volume, const int *family,
+             const int *bc_types, const double *bc_values,
+             const int *force_bc_types, const double *force_bc_values,
+             const double *stiffness_corrections, const int *bond_types,
+             const double *densities);
+
+/* Integrator.create_buffers (integrators.py:190-254): the simulation state.
+ *   nlist (N,K) i32, n_neigh (N) i32: for PD_LAYOUT_COMPACT the live bonds of
+ *   node i are nlist[i, :n_neigh[i]]; for PD_LAYOUT_SLOT every entry != -1.
+ *   nregimes == nbond_types == 1: bond_stiffness / critical_stretch point at
+ *   one double each (the PMB model) and plus_cs / regimes are ignored.
+ *   Otherwise they are (nbond_types, nregimes) tables, row-major, with
+ *   plus_cs from model.py:941-1091, and regimes (N,K) i32 or NULL (zeros);
+ *   bond_types must then have been given to pd_build (all zeros allowed).
+ *   u, ud, udd: (N,3) f64 or NULL (zeros).
+ * May be called again to restart from new state (Model.simulate does). */
+int pd_create_buffers(pd_sim *sim, const int *nlist, const int *n_neigh,
+                      const double *bond_stiffness,
+                      const double *critical_stretch, const double *plus_cs,
+                      int nregimes, int nbond_types, const double *u,
+                      const double *ud, const double *udd,
+                      const int *regimes);
+
+/* Per-node stiffness field c (N) f64, or NULL to clear it.  When set, the PMB
+ * bond stiffness of bond (i,j) is (c_i + c_j) * 0.5 instead of the scalar.
+ * No counterpart in the reference's kernels: it is the stiffness model of
+ * the probabilistic (random-field) samples, as PeriPy users build them by
+ * re-running simulate() with new bond_stiffness (model.py:1523-1542). */
+int pd_set_node_stiffness(pd_sim *sim, const double *c);
+
+/* Breaking criterion of the PMB model: 0 (default) breaks when
+ * !(|s| < s0) (peridynamics.pyx:133); 1 breaks when !(s < s0)
+ * (cl/peridynamics.cl:78). */
+int pd_set_break_mode(pd_sim *sim, int tension_only);
+
+/* Lanes per node (1, 2, 4 or 8) and threads per block (64..1024, a multiple
+ * of 64) of the step kernel.  Defaults: 1 and 256. */
+int pd_set_launch(pd_sim *sim, int lanes_per_node, int block_size);
+
+/* Integrator.__call__: one step (break, bond force, boundary conditions,
+ * integration).  Asynchronous with respect to the host. */
+int pd_step(pd_sim *sim, double displacement_bc_magnitude,
+            double force_bc_magnitude);
+
+/* nsteps calls of pd_step with magnitudes disp_mags[s], force_mags[s]
+ * (either may be NULL for zeros). */
+int pd_run(pd_sim *sim, int nsteps, const double *disp_mags,
+           const double *force_mags);
+
+/* pd_run bracketed by HIP events recorded on the library's own stream, the
+ * stream every step kernel is launched on; *elapsed_ms is the time between
+ * them (synchronous). */
+int pd_run_timed(pd_sim *sim, int nsteps, const double *disp_mags,
+                 const double *force_mags, float *elapsed_ms);
+
+/* Integrator.write (integrators.py:285-300): damage, then device -> host
+ * copies.  Any pointer may be NULL to skip that array.  nlist (N,K) and
+ * n_neigh (N) come back in the layout given to pd_build, reflecting every
+ * broken bond; damage is (family - n_neigh)/family (peridynamics.pyx:29). */
+int pd_write(pd_sim *sim, double *u, double *ud, double *udd, double *force,
+             double *body_force, double *damage, int *nlist, int *n_neigh);
+
+/* Bond regimes (N,K) i32 of the n-linear model (zeros for PMB). */
+int pd_get_regimes(pd_sim *sim, int *regimes);
+
+/* Sum over nodes of the live bond count at this moment (directed bonds). */
+int pd_live_bonds(pd_sim *sim, long long *count);
+
+/* Block the host until all work of this simulation has finished. */
+int pd_synchronize(pd_sim *sim);
+
+void pd_destroy(pd_sim *sim);
+
+/* Verification entry point: the step kernel's stretch pipeline on n bonds
+ * (r_i, r_j, r0_i, r0_j each (n,3) f64), run on `device`.  Outputs per bond:
+ * l = |r_j - r_i| and l0 = |r0_j - r0_i| (spatial.pyx:20-31, bitwise), the
+ * kernel's working stretch s, the IEEE-division stretch (l - l0)/l0
+ * (spatial.pyx:87-97) and the break decision !(|s| < s0)
+ * (peridynamics.pyx:133; !(s < s0) when tension_only). */
+int pd_bond_check(int device, int n, const double *ri, const double *rj,
+                  const double *r0i, const double *r0j, double s0,
+                  int tension_only, double *l, double *l0, double *s,
+                  double *s_exact, int *broken);
+
+/* Neighbour list of a point cloud (no counterpart kernel in the reference:
+ * model.py:536-597 builds it on the host with a KD-tree).  Node j is a
+ * neighbour of i (j != i) when ((dx*dx + dy*dy) + dz*dz) <= horizon*horizon,
+ * d = x_j - x_i.  Rows are sorted ascending by j.
+ * Pass 1: nlist == NULL fills family (N) and returns.  Pass 2: nlist (N,K)
+ * is filled with pad_value after each row's family[i] entries.  Runs on the
+ * host (no GPU needed). */
+int pd_neighbour_list(int nnodes, const double *coords, double horizon,
+                      int *family, int *nlist, int max_neighbours,
+                      int pad_value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PD_HIP_H */

@@ -1,0 +1,60 @@
+"""Copy the reference's own test data files used as golden fixtures.
+
+Run once, in the build container, where the reference tree is mounted.  It
+copies DATA files only (numpy arrays and meshes that PeriPy's test suite
+loads: peripy/test/data/*), never source.  The copies are committed next to
+this script so that the tests run without the reference present.
+
+Each fixture and the reference test that reads it:
+
+  example_mesh.vtk               conftest.py:72 (2113-node 2-D plate)
+  force_test.vtk                 test_model.py:1569 (3-node force sign model)
+  expected_volume.npy            test_model.py:263
+  expected_connectivity_basic.npz      test_model.py:317 (pre-crack, Cython)
+  expected_connectivity_crack.npz      test_regression.py:62, test_integrators.py:177
+  expected_connectivity_crack_cl.npz   test_regression.py:124, test_integrators.py:451
+  expected_displacements.npy, expected_force.npy, expected_damage.npy
+                                 test_regression.py:45-79, test_integrators.py:174-176
+  expected_*_euler_cromer.npy    test_integrators.py:575-577 (+ unreferenced velocities)
+  expected_*_velocity_verlet.npy test_integrators.py:721-727
+  expected_bc_types/values.npy, expected_force_bc_types/values.npy
+                                 test_model.py:1466-1471
+"""
+import pathlib
+import shutil
+import sys
+
+SRC = pathlib.Path("/root/reference/peripy/test/data")
+DST = pathlib.Path(__file__).resolve().parent
+
+FILES = [
+    "example_mesh.vtk",
+    "force_test.vtk",
+    "expected_volume.npy",
+    "expected_connectivity_basic.npz",
+    "expected_connectivity_crack.npz",
+    "expected_connectivity_crack_cl.npz",
+    "expected_displacements.npy",
+    "expected_force.npy",
+    "expected_damage.npy",
+    "expected_displacements_euler_cromer.npy",
+    "expected_force_euler_cromer.npy",
+    "expected_damage_euler_cromer.npy",
+    "expected_velocities_euler_cromer.npy",
+    "expected_displacements_velocity_verlet.npy",
+    "expected_velocities_velocity_verlet.npy",
+    "expected_accelerations_velocity_verlet.npy",
+    "expected_force_velocity_verlet.npy",
+    "expected_damage_velocity_verlet.npy",
+    "expected_bc_types.npy",
+    "expected_bc_values.npy",
+    "expected_force_bc_types.npy",
+    "expected_force_bc_values.npy",
+]
+
+if __name__ == "__main__":
+    if not SRC.is_dir():
+        sys.exit(f"{SRC} not found: the fixtures are already committed.")
+    for name in FILES:
+        shutil.copyfile(SRC / name, DST / name)
+        print("copied", name)
