@@ -83,7 +83,7 @@ namespace pdk {
 constexpr uint32_t DEAD = 0x80000000u;
 constexpr uint32_t NEWB = 0x40000000u;
 constexpr uint32_t KEYM = 0x3FFFFFFFu;
-constexpr int MAX_NODES = 0x3FFFFFFF;
+constexpr int MAX_NODES = 1 << 26;  // gathers use a 32-bit byte offset j*64
 constexpr double GUARD = 1e-10;
 constexpr int MAX_TAB = 256;  // nbond_types * nregimes
 constexpr int PAD_CHUNKS = 8;
@@ -828,7 +828,7 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
     if (layout != PD_LAYOUT_COMPACT && layout != PD_LAYOUT_SLOT)
         return fail(PD_E_ARG, "layout must be 0 (compact) or 1 (slot)");
     if (nnodes <= 0 || nnodes > MAX_NODES)
-        return fail(PD_E_LIMIT, "nnodes out of range");
+        return fail(PD_E_LIMIT, "nnodes must be in 1..2^26");
     if (max_neighbours <= 0 || max_neighbours > 65535)
         return fail(PD_E_LIMIT, "max_neighbours must be in 1..65535");
     if (!coords || !volume || !family || !bc_types || !bc_values ||
