@@ -104,8 +104,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="beam_vv",
                     choices=["beam_vv", "composite_ec", "bar_euler"])
-    ap.add_argument("--lanes-per-node", type=int, default=1)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--lanes-per-node", type=int, default=4)
+    ap.add_argument("--tile-nodes", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 to skip)")
     args = ap.parse_args()
@@ -121,12 +121,13 @@ def main():
         raise SystemExit("bench.py needs a HIP GPU")
     t_setup = time.perf_counter()
     w = getattr(configs, args.workload)()
-    integ = configs.make_integrator(w)
-    integ.set_launch(args.lanes_per_node, args.block)
+    integ = configs.make_integrator(w, lanes_per_node=args.lanes_per_node,
+                                    tile_nodes=args.tile_nodes)
+    tiles = integ.tile_stats()
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] {w.name}: N={w.nnodes} L={w.live_bonds()} "
-          f"K={w.max_neighbours} setup {setup_s:.1f}s", file=sys.stderr,
-          flush=True)
+          f"K={w.max_neighbours} setup {setup_s:.1f}s tiles {tiles}",
+          file=sys.stderr, flush=True)
 
     dm, fm = w.magnitudes(args.warmup + args.steps)
     integ.run(args.warmup, dm[:args.warmup], fm[:args.warmup])
@@ -172,7 +173,9 @@ def main():
             "max_neighbours": w.max_neighbours,
             "integrator": w.integrator,
             "lanes_per_node": args.lanes_per_node,
-            "block": args.block,
+            "tile_nodes": args.tile_nodes,
+            "tiles": tiles["ntiles"],
+            "halo_per_node": tiles["halo_total"] / w.nnodes,
             "device": f"{name} ({arch})",
         },
         "roofline": {

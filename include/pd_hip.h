@@ -110,9 +110,20 @@ int pd_set_node_stiffness(pd_sim *sim, const double *c);
  * (cl/peridynamics.cl:78). */
 int pd_set_break_mode(pd_sim *sim, int tension_only);
 
-/* Lanes per node (1, 2, 4 or 8) and threads per block (64..1024, a multiple
- * of 64) of the step kernel.  Defaults: 1 and 256. */
-int pd_set_launch(pd_sim *sim, int lanes_per_node, int block_size);
+/* Launch shape of the step kernel: lanes per node (1, 2, 4 or 8) and the
+ * largest number of nodes per tile (a multiple of 64, with
+ * tile_nodes * lanes_per_node <= 1024; one workgroup runs one tile).
+ * Defaults: 4 and 128.  lanes_per_node may change at any time; tile_nodes
+ * must be set before pd_create_buffers, which builds the tiles.  A tile's
+ * halo (its nodes and all their neighbours) is capped at 1280 entries (1184
+ * for n-linear models), so a node may have at most 1279 (1183) bonds. */
+int pd_set_launch(pd_sim *sim, int lanes_per_node, int tile_nodes);
+
+/* Tiling of the current state: number of tiles, halo capacity per tile
+ * (from the device's LDS per workgroup), largest halo of any tile (sets the
+ * LDS each workgroup requests), total halo entries, bond slots. */
+int pd_tile_stats(pd_sim *sim, int *ntiles, int *hmax, int *hstride,
+                  long long *halo_total, long long *slots);
 
 /* Integrator.__call__: one step (break, bond force, boundary conditions,
  * integration).  Asynchronous with respect to the host. */

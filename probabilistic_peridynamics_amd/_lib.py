@@ -54,6 +54,7 @@ SIGNATURES = {
     "pd_get_regimes": (ctypes.c_int, [_SIM, _I]),
     "pd_live_bonds": (ctypes.c_int, [_SIM, _LL]),
     "pd_synchronize": (ctypes.c_int, [_SIM]),
+    "pd_tile_stats": (ctypes.c_int, [_SIM, _I, _I, _I, _LL, _LL]),
     "pd_destroy": (None, [_SIM]),
     "pd_bond_check": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_double,

@@ -239,8 +239,9 @@ def composite_ec(shape=(470, 47, 47), seed=2):
         extra={"dx": dx, "shape": shape, "seed": seed})
 
 
-def make_integrator(w, context=None):
-    """A built integrator with the workload's state uploaded."""
+def make_integrator(w, context=None, lanes_per_node=None, tile_nodes=None):
+    """A built integrator with the workload's state uploaded (launch shape
+    optional: see Integrator.set_launch)."""
     from . import integrators as it
     if w.integrator == "euler":
         integ = it.EulerHIP(w.dt, context=context)
@@ -251,6 +252,8 @@ def make_integrator(w, context=None):
     integ.build(w.nnodes, 3, w.max_neighbours, w.coords, w.volume, w.family,
                 w.bc_types, w.bc_values, w.force_bc_types, w.force_bc_values,
                 None, w.bond_types, w.densities)
+    if lanes_per_node is not None or tile_nodes is not None:
+        integ.set_launch(lanes_per_node or 4, tile_nodes or 128)
     n = w.nnodes
     z = np.zeros((n, 3))
     integ.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,

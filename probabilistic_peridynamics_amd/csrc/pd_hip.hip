@@ -9,27 +9,44 @@
 // the step (the break-before-force order of peripy/integrators.py:339-349 and
 // the single bond_force pass of cl/peridynamics.cl both read u_n only).
 //
-// Data layout in HBM (N nodes, Np = N rounded up to 64, Kc = longest row,
-// C = ceil(Kc/4) + 8 chunk rows):
+// Tiles.  The nodes are cut into tiles: runs of up to `tile_nodes` nodes in
+// Morton order of their initial coordinates, grown while the tile's halo --
+// the tile's nodes and all their neighbours -- fits in LDS (1280 entries,
+// 80 KiB, so two workgroups share a CU and one stages its halo while the
+// other computes).  Within a tile the nodes are ordered by descending bond
+// count, so a wavefront's nodes run loops of nearly equal length.  One
+// workgroup runs one tile: it first copies the halo's records into LDS with
+// coalesced loads (each record once), then every bond gathers its neighbour
+// from LDS.
+// A direct global gather would fetch each 64-byte record with four 16-byte
+// loads from 64 different records per instruction, and the vector L1 then
+// handles one 64-byte sector per clock: that, not HBM, was the bound.
+// Tiles are mapped to workgroups so that each XCD (blockIdx % 8) runs a
+// contiguous range of tiles, whose halos overlap in that XCD's L2.
+//
+// Data layout in HBM (N nodes, S bond slots):
 //   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, V | rx, ry, rz, c) with
 //              r = r0 + u (ping-pong; the r0/V half is static in both) and
-//              c the per-node stiffness (or 0).  A neighbour gather reads one
-//              64-byte-aligned record.
-//   nbr[C*Np*4] int32, neighbour-major in chunks of 4: bond k of node i is at
-//              ((k/4)*Np + i)*4 + k%4, so one 16-byte load gives a lane four
-//              consecutive bonds and a wavefront reads 1 KiB contiguously.
-//              Rows are sorted ascending by j (the k-th neighbours of
-//              consecutive nodes are then close in memory).  Bit 31 marks a
-//              broken bond (DEAD), bit 30 one broken in the current step
-//              (NEW, compact layout only); bits 0-29 hold j.  The 8 extra
-//              chunk rows are DEAD padding for the one-chunk-ahead prefetch.
-//   cnt[N]     number of bond slots of node i
+//              c the per-node stiffness (or 0).
+//   tiles[T]   TileInfo: node range in tnode[], halo range in halo[], first
+//              chunk of the tile's bond slots, chunk rows.
+//   tnode      node ids in tile order; halo: ascending node ids per tile.
+//   bond slots, in chunks of 4: chunk (c, l) of a tile with n nodes is
+//              chunk_off + c*n + l and holds bonds 4c..4c+3 of its l-th node,
+//              rows sorted ascending by j; a wavefront reads its nodes' c-th
+//              chunks contiguously.  Each tile has 8 padding chunk rows so
+//              that the one-chunk-ahead prefetch stays in bounds.
+//     loc   u16  halo-local index of j; 0xFFFF: broken or empty (hot loop)
+//     gidx  i32  j, with bit 31 DEAD and bit 30 NEW (broken in this step,
+//                compact layout) -- read only on breaks and exports
+//     pos   u16  reference column of the slot (exports)
+//     corr  f64, state u16 (bond type << 8 | regime): optional
+//   chunk0[N], stride[N]: node i's chunk 0 and its tile's node count, so
+//              slot(i, k) = (chunk0 + (k/4)*stride)*4 + k%4
 //   u, ud, udd, force, body_force (N,3) f64 in the reference's layout
 //   flags[N]   bit d: displacement BC on DOF d; bit 3+d: force BC on DOF d
 //   bcv, fbcv, dens (N,3) f64, read only for flagged DOFs / dynamic
 //              integrators
-//   corr (f64), state (u16: bond type << 8 | regime), pos (u16: reference
-//              column of the slot): per-slot, in nbr's order
 //   nlist_ref (N,K) i32 + n_neigh: the reference-layout list of the compact
 //              (Cython) layout, maintained by in-kernel compaction.
 //
@@ -83,10 +100,21 @@ namespace pdk {
 constexpr uint32_t DEAD = 0x80000000u;
 constexpr uint32_t NEWB = 0x40000000u;
 constexpr uint32_t KEYM = 0x3FFFFFFFu;
-constexpr int MAX_NODES = 1 << 26;  // gathers use a 32-bit byte offset j*64
+constexpr unsigned short LOC_DEAD = 0xFFFF;
+constexpr int MAX_NODES = 1 << 26;
 constexpr double GUARD = 1e-10;
 constexpr int MAX_TAB = 256;  // nbond_types * nregimes
 constexpr int PAD_CHUNKS = 8;
+constexpr int MAX_LPN = 8;
+constexpr int XCDS = 8;
+// Halo capacity of a tile: entries of 4 x 16 bytes, stored piece-major (piece
+// q of entry e at byte q*16*cap + 16*e), so that one address register and
+// immediate offsets reach all four pieces and random entries spread over all
+// LDS banks.  4*16*cap bytes (plus the n-linear tables) stay within 80 KiB,
+// which lets two workgroups share a CU's 160 KiB.
+constexpr int LDS_CAP_PMB = 1280;
+constexpr int LDS_CAP_NLIN = 1184;
+constexpr unsigned LDS_BYTES = 81920;
 
 // damage models of the step kernel
 constexpr int DM_PMB = 0;    // scalar stiffness
@@ -98,28 +126,40 @@ struct alignas(64) NodeRec {
     double4 b;  // rx, ry, rz, c
 };
 
+struct TileInfo {
+    int node_off, nodes, halo_off, halo;
+    long long chunk_off;
+    int nch, pad;
+};
+
 struct StepArgs {
-    int n, np, K;
+    int n, K, hstride, ntiles, tiles_per_xcd;
     int integrator, layout, has_corr;
     unsigned absmask;  // 0x7fffffff: break on |s|; 0xffffffff: on s
     int nregimes, ntab;
     double dt, damping, s0h, s0h_guard, k4;
     const NodeRec *rec_old;
     NodeRec *rec_new;
+    const TileInfo *tiles;
+    const int *tnode, *halo;
     double *u, *ud, *udd, *force, *body_force;
     const int *flags;
     const double *bcv, *fbcv, *dens;
     const int *cnt;
-    int *nbr;
+    unsigned short *loc;
+    int *gidx;
     const double *corr;
     unsigned short *state;
     const double *k_tab, *s_tab, *pc_tab;
+    const long long *chunk0;
+    const int *stride;
     int *nlist_ref, *n_neigh;
 };
 
-__host__ __device__ __forceinline__ size_t sidx(int k, uint32_t i, int np)
+__host__ __device__ __forceinline__ size_t slot_of(long long chunk0,
+                                                   int stride, int k)
 {
-    return ((size_t)(k >> 2) * (size_t)np + i) * 4 + (k & 3);
+    return (size_t)(chunk0 + (long long)(k >> 2) * stride) * 4 + (k & 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -157,7 +197,7 @@ __device__ __forceinline__ double sign_mask(double x, unsigned mask)
                             __double2loint(x));
 }
 
-__device__ __forceinline__ uint32_t ld_nbr(const int *p)
+__device__ __forceinline__ uint32_t ld_slot(const int *p)
 {
     return (uint32_t)__hip_atomic_load(p, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -165,12 +205,13 @@ __device__ __forceinline__ uint32_t ld_nbr(const int *p)
 
 // Is j a bond of node i that broke in this step?  Binary search of the
 // (ascending) keys of i's slots.
-__device__ bool broke_now(const int *nbr, int np, uint32_t i, int cnt, int j)
+__device__ bool broke_now(const int *gidx, long long c0, int stride, int cnt,
+                          int j)
 {
     int lo = 0, hi = cnt - 1;
     while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
-        const uint32_t e = ld_nbr(&nbr[sidx(mid, i, np)]);
+        const uint32_t e = ld_slot(&gidx[slot_of(c0, stride, mid)]);
         const int key = (int)(e & KEYM);
         if (key == j)
             return (e & NEWB) != 0;
@@ -189,15 +230,15 @@ __device__ bool broke_now(const int *nbr, int np, uint32_t i, int cnt, int j)
 // i's own sweep replaces each broken j > i by the last live entry without
 // advancing.  All of it depends only on row i and the set of bonds of i that
 // broke this step, which are the NEW slots of node i.
-__device__ __noinline__ void compact_row(int *nbr, int np, int *nlist_ref,
-                                         int K, int *n_neigh, uint32_t i,
-                                         int cnt)
+__device__ __noinline__ void compact_row(int *gidx, long long c0, int stride,
+                                         int *nlist_ref, int K, int *n_neigh,
+                                         uint32_t i, int cnt)
 {
     __threadfence();
     int *R = nlist_ref + (size_t)i * K;
     int n = n_neigh[i];
     for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = ld_nbr(&nbr[sidx(k, i, np)]);
+        const uint32_t e = ld_slot(&gidx[slot_of(c0, stride, k)]);
         const int j = (int)(e & KEYM);
         if ((e & NEWB) && j < (int)i) {
             for (int p = 0; p < n; ++p) {
@@ -212,7 +253,7 @@ __device__ __noinline__ void compact_row(int *nbr, int np, int *nlist_ref,
     int p = 0;
     while (p < n) {
         const int j = R[p];
-        if (j > (int)i && broke_now(nbr, np, i, cnt, j)) {
+        if (j > (int)i && broke_now(gidx, c0, stride, cnt, j)) {
             R[p] = R[n - 1];
             --n;
         } else {
@@ -220,8 +261,8 @@ __device__ __noinline__ void compact_row(int *nbr, int np, int *nlist_ref,
         }
     }
     for (int k = 0; k < cnt; ++k) {
-        int *q = &nbr[sidx(k, i, np)];
-        const uint32_t e = ld_nbr(q);
+        int *q = &gidx[slot_of(c0, stride, k)];
+        const uint32_t e = ld_slot(q);
         if (e & NEWB)
             *q = (int)(e & ~NEWB);
     }
@@ -232,147 +273,233 @@ __device__ __noinline__ void compact_row(int *nbr, int np, int *nlist_ref,
 // the step kernel
 // ---------------------------------------------------------------------------
 
-// One bond (i, j = e) in slot `slot` (an index into nbr / corr / state).
+// A chunk of four bonds of node i.  e4 holds the neighbours' halo indices
+// (LOC_DEAD: broken or empty), s0 the slot index of the first bond.  The
+// pieces of halo entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, c), with
+// w = 4*k*V for DM_PMB and w = V otherwise.  The four
+// bonds' reads are issued first and their arithmetic is straight-line code
+// (a dead slot is computed on entry 0 and masked), so the four chains
+// interleave; only the exact-division path -- a stretch near a threshold,
+// or a square root outside the core's range -- and bond breaking branch.
 template <int DM>
-__device__ __forceinline__ void bond(const StepArgs &a, const char *rb,
-                                     const double4 &r0i, const double4 &ri,
-                                     int e, size_t slot, const double *ts,
-                                     const double *tk, const double *tp,
-                                     double &fx, double &fy, double &fz,
-                                     int &nbroken)
+constexpr unsigned piece_bytes()
 {
-    if (e < 0)
-        return;
-    const NodeRec *pj =
-        reinterpret_cast<const NodeRec *>(rb + ((uint32_t)e << 6));
-    const double4 r0j = pj->a;
-    const double4 rj = pj->b;
-
-    // spatial.pyx:20-31 with r = r0 + u: dr = r_j - r_i
-    const double dx = rj.x - ri.x;
-    const double dy = rj.y - ri.y;
-    const double dz = rj.z - ri.z;
-    const double q = (dx * dx + dy * dy) + dz * dz;
-    const double ex = r0j.x - r0i.x;
-    const double ey = r0j.y - r0i.y;
-    const double ez = r0j.z - r0i.z;
-    const double q0 = (ex * ex + ey * ey) + ez * ez;
-
-    double l, hl, l0, h0;
-    sqrt_core(q, l, hl);
-    sqrt_core(q0, l0, h0);
-    double dl = l - l0;
-    double sh = dl * h0;  // s/2, from h0 ~ 0.5/l0
-    const bool fast = core_range(q) & core_range(q0);
-
-    double g;
-    if (DM != DM_NLIN) {
-        double as = sign_mask(sh, a.absmask);
-        if (!fast || fabs(as - a.s0h) <= a.s0h_guard) {
-            // rare: outside the core's range, or near s0
-            if (!fast) {
-                l = sqrt(q);
-                l0 = sqrt(q0);
-                hl = 0.5 / l;
-                dl = l - l0;
-            }
-            sh = (dl / l0) * 0.5;
-            as = sign_mask(sh, a.absmask);
-        }
-        if (!(as < a.s0h)) {
-            a.nbr[slot] = (int)((uint32_t)e | DEAD |
-                                (a.layout == PD_LAYOUT_COMPACT ? NEWB : 0u));
-            ++nbroken;
-            return;
-        }
-        // s*k/l*V_j = (s/2)(0.5/l)(4k) V_j
-        const double k4 = DM == DM_FIELD ? (ri.w + rj.w) * 2.0 : a.k4;
-        g = sh * hl * k4 * r0j.w;
-    } else {
-        // cl/peridynamics.cl:326-378 (bond_force3/4) with thresholds halved
-        const uint32_t st = a.state[slot];
-        const int t = (int)(st >> 8);
-        const int reg = (int)(st & 0xffu);
-        const int base = t * a.nregimes;
-        const double c_hi = ts[base + reg];
-        const double c_lo = reg > 0 ? ts[base + reg - 1] : -__builtin_inf();
-        if (!fast || fabs(sh - c_hi) <= GUARD * fabs(c_hi) ||
-            fabs(sh - c_lo) <= GUARD * fabs(c_lo)) {
-            if (!fast) {
-                l = sqrt(q);
-                l0 = sqrt(q0);
-                hl = 0.5 / l;
-                dl = l - l0;
-            }
-            sh = (dl / l0) * 0.5;
-        }
-        int nreg = reg;
-        if (sh < c_hi) {
-            if (reg > 0 && sh < c_lo)
-                nreg = reg - 1;
-        } else {
-            nreg = reg + 1;
-        }
-        if (nreg != reg)
-            a.state[slot] = (unsigned short)((t << 8) | nreg);
-        if (nreg >= a.nregimes) {
-            a.nbr[slot] = (int)((uint32_t)e | DEAD);
-            ++nbroken;
-            return;
-        }
-        const double fn = __builtin_fma(sh + sh, tk[base + nreg], tp[base + nreg]);
-        g = fn * (hl + hl) * r0j.w;
-    }
-    if (a.has_corr)
-        g *= a.corr[slot];
-    fx = __builtin_fma(g, dx, fx);
-    fy = __builtin_fma(g, dy, fy);
-    fz = __builtin_fma(g, dz, fz);
+    return 16u * (DM == DM_NLIN ? LDS_CAP_NLIN : LDS_CAP_PMB);
 }
 
-// DM: DM_PMB, DM_NLIN or DM_FIELD.  LPN lanes share a node, taking its
-// 4-bond chunks round-robin, and reduce with cross-lane shuffles.
-template <int DM, int LPN>
-__global__ __launch_bounds__(256) void k_step(StepArgs a, double dmag,
-                                              double fmag)
+template <int DM>
+__device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
+                                       const double4 &r0i, const double4 &ri,
+                                       ushort4 e4, size_t s0,
+                                       const double *ts, const double *tk,
+                                       const double *tp, double &fx,
+                                       double &fy, double &fz, int &nbroken)
 {
-    __shared__ double tk[DM == DM_NLIN ? MAX_TAB : 1],
-        ts[DM == DM_NLIN ? MAX_TAB : 1], tp[DM == DM_NLIN ? MAX_TAB : 1];
+    const unsigned ev[4] = {e4.x, e4.y, e4.z, e4.w};
+    unsigned short st[4] = {0, 0, 0, 0};
+    if (DM == DM_NLIN) {
+        const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.state + s0);
+        st[0] = s4.x;
+        st[1] = s4.y;
+        st[2] = s4.z;
+        st[3] = s4.w;
+    }
+    // per bond, what the force needs: dr, 0.5/l, s/2 and the weight
+    double dx[4], dy[4], dz[4], hl[4], sh[4], w[4];
+    bool live[4], slow[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        live[q] = ev[q] != LOC_DEAD;
+        const unsigned idx = live[q] ? ev[q] : 0u;
+        const char *pe = ldsb + 16u * idx;
+        constexpr unsigned PB = piece_bytes<DM>();
+        const double2 p0 = *reinterpret_cast<const double2 *>(pe);
+        const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
+        const double2 p2 = *reinterpret_cast<const double2 *>(pe + 2 * PB);
+        const double2 p3 = *reinterpret_cast<const double2 *>(pe + 3 * PB);
+        // spatial.pyx:20-31 with r = r0 + u: dr = r_j - r_i
+        dx[q] = p2.x - ri.x;
+        dy[q] = p2.y - ri.y;
+        dz[q] = p3.x - ri.z;
+        const double qq = (dx[q] * dx[q] + dy[q] * dy[q]) + dz[q] * dz[q];
+        const double ex = p0.x - r0i.x;
+        const double ey = p0.y - r0i.y;
+        const double ez = p1.x - r0i.z;
+        const double q0 = (ex * ex + ey * ey) + ez * ez;
+        double l, l0, h0;
+        sqrt_core(qq, l, hl[q]);
+        sqrt_core(q0, l0, h0);
+        sh[q] = (l - l0) * h0;  // s/2, from h0 ~ 0.5/l0
+        // k = (c_i + c_j)/2 for a field: s k/l V_j = (s/2)(0.5/l)(2(c_i+c_j)V_j)
+        w[q] = DM == DM_FIELD ? (ri.w + p3.y) * 2.0 * p1.y : p1.y;
+        const bool fast = core_range(qq) & core_range(q0);
+        bool near;
+        if (DM != DM_NLIN) {
+            near = fabs(sign_mask(sh[q], a.absmask) - a.s0h) <= a.s0h_guard;
+        } else {
+            const int base = (st[q] >> 8) * a.nregimes;
+            const int reg = st[q] & 0xff;
+            const double c_hi = ts[base + reg];
+            const double c_lo = reg > 0 ? ts[base + reg - 1] : -__builtin_inf();
+            near = fabs(sh[q] - c_hi) <= GUARD * fabs(c_hi) ||
+                   fabs(sh[q] - c_lo) <= GUARD * fabs(c_lo);
+        }
+        slow[q] = live[q] & (!fast | near);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (slow[q]) {
+            // rare: near a threshold, or a square root outside the core's
+            // range: IEEE square roots and division (spatial.pyx:87-97)
+            const char *pe = ldsb + 16u * ev[q];
+            const double2 p0 = *reinterpret_cast<const double2 *>(pe);
+            const double2 p1 =
+                *reinterpret_cast<const double2 *>(pe + piece_bytes<DM>());
+            const double qq = (dx[q] * dx[q] + dy[q] * dy[q]) + dz[q] * dz[q];
+            const double ex = p0.x - r0i.x;
+            const double ey = p0.y - r0i.y;
+            const double ez = p1.x - r0i.z;
+            const double q0 = (ex * ex + ey * ey) + ez * ez;
+            const double l = sqrt(qq), l0 = sqrt(q0);
+            if (!(core_range(qq) & core_range(q0)))
+                hl[q] = 0.5 / l;
+            sh[q] = ((l - l0) / l0) * 0.5;
+        }
+    }
+    double cr[4] = {1.0, 1.0, 1.0, 1.0};
+    if (a.has_corr) {
+        const double4 c4 = *reinterpret_cast<const double4 *>(a.corr + s0);
+        cr[0] = c4.x;
+        cr[1] = c4.y;
+        cr[2] = c4.z;
+        cr[3] = c4.w;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!live[q])
+            continue;
+        double g;
+        if (DM != DM_NLIN) {
+            if (!(sign_mask(sh[q], a.absmask) < a.s0h)) {
+                a.loc[s0 + q] = LOC_DEAD;
+                a.gidx[s0 + q] |=
+                    (int)(DEAD | (a.layout == PD_LAYOUT_COMPACT ? NEWB : 0u));
+                ++nbroken;
+                continue;
+            }
+            // s k/l V_j = (s/2)(0.5/l)(4 k V_j)
+            g = sh[q] * hl[q] * w[q];
+        } else {
+            // cl/peridynamics.cl:326-378 (bond_force3/4), thresholds halved
+            const int t = st[q] >> 8;
+            const int reg = st[q] & 0xff;
+            const int base = t * a.nregimes;
+            const double c_hi = ts[base + reg];
+            const double c_lo = reg > 0 ? ts[base + reg - 1] : -__builtin_inf();
+            int nreg = reg;
+            if (sh[q] < c_hi) {
+                if (reg > 0 && sh[q] < c_lo)
+                    nreg = reg - 1;
+            } else {
+                nreg = reg + 1;
+            }
+            if (nreg != reg)
+                a.state[s0 + q] = (unsigned short)((t << 8) | nreg);
+            if (nreg >= a.nregimes) {
+                a.loc[s0 + q] = LOC_DEAD;
+                a.gidx[s0 + q] |= (int)DEAD;
+                ++nbroken;
+                continue;
+            }
+            const double fn =
+                __builtin_fma(sh[q] + sh[q], tk[base + nreg], tp[base + nreg]);
+            g = fn * (hl[q] + hl[q]) * w[q];
+        }
+        g *= cr[q];
+        fx = __builtin_fma(g, dx[q], fx);
+        fy = __builtin_fma(g, dy[q], fy);
+        fz = __builtin_fma(g, dz[q], fz);
+    }
+}
+
+// DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
+// node taking its 4-bond chunks round-robin and reducing with cross-lane
+// shuffles.  LDS (dynamic, LDS_BYTES): the halo, four pieces of
+// piece_bytes<DM>() each, then for DM_NLIN the three damage-model tables.
+template <int DM, int LPN>
+__global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
+                                               double fmag)
+{
+    extern __shared__ double2 lds[];
+    const int b = blockIdx.x;
+    const int tile = (b % XCDS) * a.tiles_per_xcd + b / XCDS;
+    if (tile >= a.ntiles)
+        return;
+    const TileInfo ti = a.tiles[tile];
+    char *ldsb = reinterpret_cast<char *>(lds);
+
+    // stage the halo: piece x copies 16-byte piece x%4 of record x/4, SP
+    // pieces per thread in flight at once (loads clamped in bounds, stores
+    // predicated); the r0/V piece of DM_PMB becomes (r0z, 4 k V)
+    constexpr int SP = 6;
+    constexpr unsigned PB = piece_bytes<DM>();
+    const int np = 4 * ti.halo;
+    for (int x0 = threadIdx.x; x0 < np; x0 += SP * blockDim.x) {
+        double2 v[SP];
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            const int x = min(x0 + k * (int)blockDim.x, np - 1);
+            const int g = a.halo[ti.halo_off + (x >> 2)];
+            v[k] = reinterpret_cast<const double2 *>(a.rec_old + g)[x & 3];
+        }
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            const int x = x0 + k * (int)blockDim.x;
+            if (x < np) {
+                double2 w = v[k];
+                if (DM == DM_PMB && (x & 3) == 1)
+                    w.y = a.k4 * w.y;
+                *reinterpret_cast<double2 *>(ldsb + (x & 3) * PB +
+                                             16u * (unsigned)(x >> 2)) = w;
+            }
+        }
+    }
+    double *ts = reinterpret_cast<double *>(ldsb + 4 * PB);
+    double *tk = ts + MAX_TAB, *tp = tk + MAX_TAB;
     if (DM == DM_NLIN) {
         for (int t = threadIdx.x; t < a.ntab; t += blockDim.x) {
             tk[t] = a.k_tab[t];
             ts[t] = a.s_tab[t] * 0.5;
             tp[t] = a.pc_tab[t];
         }
-        __syncthreads();
     }
+    __syncthreads();
 
-    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = gt / LPN;
-    const int sub = (int)(gt % LPN);
-    const bool valid = i < (uint32_t)a.n;
+    const int l = threadIdx.x / LPN;
+    const int sub = threadIdx.x % LPN;
+    const bool valid = l < ti.nodes;
 
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int nbroken = 0;
     int cnt = 0;
+    uint32_t i = 0;
     double4 r0i = make_double4(0.0, 0.0, 0.0, 0.0), ri = r0i;
     if (valid) {
+        i = (uint32_t)a.tnode[ti.node_off + l];
         const NodeRec *pi = a.rec_old + i;
         r0i = pi->a;
         ri = pi->b;
         cnt = a.cnt[i];
-        const char *rb = reinterpret_cast<const char *>(a.rec_old);
         const int nch = (cnt + 3) >> 2;
-        const int4 *nb4 = reinterpret_cast<const int4 *>(a.nbr);
-        int4 cur = nb4[(size_t)sub * a.np + i];
+        const ushort4 *L4 = reinterpret_cast<const ushort4 *>(a.loc);
+        const long long c0 = ti.chunk_off + l;
+        ushort4 cur = L4[c0 + (long long)sub * ti.nodes];
         for (int c = sub; c < nch; c += LPN) {
-            // prefetch the next chunk (in bounds: PAD_CHUNKS rows of DEAD)
-            const int4 nxt = nb4[(size_t)(c + LPN) * a.np + i];
-            const size_t s0 = ((size_t)c * a.np + i) * 4;
-            bond<DM>(a, rb, r0i, ri, cur.x, s0 + 0, ts, tk, tp, fx, fy, fz, nbroken);
-            bond<DM>(a, rb, r0i, ri, cur.y, s0 + 1, ts, tk, tp, fx, fy, fz, nbroken);
-            bond<DM>(a, rb, r0i, ri, cur.z, s0 + 2, ts, tk, tp, fx, fy, fz, nbroken);
-            bond<DM>(a, rb, r0i, ri, cur.w, s0 + 3, ts, tk, tp, fx, fy, fz, nbroken);
+            // prefetch the next chunk (in bounds: PAD_CHUNKS rows per tile)
+            const ushort4 nxt = L4[c0 + (long long)(c + LPN) * ti.nodes];
+            const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
+            chunk4<DM>(a, ldsb, r0i, ri, cur, s0, ts, tk, tp, fx, fy, fz,
+                       nbroken);
             cur = nxt;
         }
     }
@@ -388,55 +515,56 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a, double dmag,
         return;
 
     if (a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
-        compact_row(a.nbr, a.np, a.nlist_ref, a.K, a.n_neigh, i, cnt);
+        compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
+                    a.n_neigh, i, cnt);
 
     // force boundary condition after the bond sum (peridynamics.pyx:93-97,
     // cl/peridynamics.cl:117-129)
     const int fl = a.flags[i];
-    const size_t b = 3 * (size_t)i;
+    const size_t b3 = 3 * (size_t)i;
     const double bf[3] = {fx, fy, fz};
     double F[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        F[d] = (fl >> (3 + d)) & 1 ? bf[d] + fmag * a.fbcv[b + d] : bf[d];
-        a.body_force[b + d] = bf[d];
-        a.force[b + d] = F[d];
+        F[d] = (fl >> (3 + d)) & 1 ? bf[d] + fmag * a.fbcv[b3 + d] : bf[d];
+        a.body_force[b3 + d] = bf[d];
+        a.force[b3 + d] = F[d];
     }
 
-    double u[3] = {a.u[b], a.u[b + 1], a.u[b + 2]};
+    double u[3] = {a.u[b3], a.u[b3 + 1], a.u[b3 + 2]};
     if (a.integrator == PD_EULER) {
         // peridynamics.pyx:179-185, cl/euler.cl:21-23
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b + d] : u[d] + a.dt * F[d];
+            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d] : u[d] + a.dt * F[d];
     } else if (a.integrator == PD_EULER_CROMER) {
         // cl/euler_cromer.cl:29-34
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double ud = a.ud[b + d];
-            const double uddi = (F[d] - a.damping * ud) / a.dens[b + d];
+            double ud = a.ud[b3 + d];
+            const double uddi = (F[d] - a.damping * ud) / a.dens[b3 + d];
             ud = ud + uddi * a.dt;
-            a.udd[b + d] = uddi;
-            a.ud[b + d] = ud;
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b + d] : u[d] + a.dt * ud;
+            a.udd[b3 + d] = uddi;
+            a.ud[b3 + d] = ud;
+            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d] : u[d] + a.dt * ud;
         }
     } else {
         // cl/velocity_verlet.cl:29-35
         const double hdt = a.dt / 2;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double ud1 = a.ud[b + d] + hdt * a.udd[b + d];
-            const double udd1 = (F[d] - a.damping * ud1) / a.dens[b + d];
+            const double ud1 = a.ud[b3 + d] + hdt * a.udd[b3 + d];
+            const double udd1 = (F[d] - a.damping * ud1) / a.dens[b3 + d];
             const double ud = ud1 + hdt * udd1;
-            a.ud[b + d] = ud;
-            a.udd[b + d] = udd1;
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b + d]
+            a.ud[b3 + d] = ud;
+            a.udd[b3 + d] = udd1;
+            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d]
                                  : u[d] + a.dt * (ud + hdt * udd1);
         }
     }
-    a.u[b] = u[0];
-    a.u[b + 1] = u[1];
-    a.u[b + 2] = u[2];
+    a.u[b3] = u[0];
+    a.u[b3 + 1] = u[1];
+    a.u[b3 + 2] = u[2];
     a.rec_new[i].b =
         make_double4(r0i.x + u[0], r0i.y + u[1], r0i.z + u[2], ri.w);
 }
@@ -507,8 +635,9 @@ __global__ void k_records(int n, const double *r0, const double *vol,
 // Live-slot count per node, the damage (family - n)/family
 // (peridynamics.pyx:29), and, for the slot layout, the reference-layout
 // neighbour list: -1 everywhere but at the live slots' columns.
-__global__ void k_export(int n, int np, int K, int layout, const int *cnt,
-                         const int *nbr, const unsigned short *pos,
+__global__ void k_export(int n, int K, int layout, const int *cnt,
+                         const long long *chunk0, const int *stride,
+                         const int *gidx, const unsigned short *pos,
                          const int *family, const int *n_neigh_c,
                          int *nlist_out, int *n_neigh_out, double *damage)
 {
@@ -523,8 +652,8 @@ __global__ void k_export(int n, int np, int K, int layout, const int *cnt,
         live = 0;
         const int c = cnt[i];
         for (int k = 0; k < c; ++k) {
-            const size_t s = sidx(k, (uint32_t)i, np);
-            const int e = nbr[s];
+            const size_t s = slot_of(chunk0[i], stride[i], k);
+            const int e = gidx[s];
             if (e >= 0) {
                 row[pos[s]] = e;
                 ++live;
@@ -538,7 +667,8 @@ __global__ void k_export(int n, int np, int K, int layout, const int *cnt,
     damage[i] = (f - (double)live) / f;
 }
 
-__global__ void k_export_regimes(int n, int np, int K, const int *cnt,
+__global__ void k_export_regimes(int n, int K, const int *cnt,
+                                 const long long *chunk0, const int *stride,
                                  const unsigned short *pos,
                                  const unsigned short *state, int *out)
 {
@@ -547,12 +677,13 @@ __global__ void k_export_regimes(int n, int np, int K, const int *cnt,
         return;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
-        const size_t s = sidx(k, (uint32_t)i, np);
+        const size_t s = slot_of(chunk0[i], stride[i], k);
         out[(size_t)i * K + pos[s]] = state[s] & 0xff;
     }
 }
 
-__global__ void k_live(int n, int np, const int *cnt, const int *nbr,
+__global__ void k_live(int n, const int *cnt, const long long *chunk0,
+                       const int *stride, const int *gidx,
                        unsigned long long *total)
 {
     __shared__ unsigned long long part[256];
@@ -561,7 +692,7 @@ __global__ void k_live(int n, int np, const int *cnt, const int *nbr,
     if (i < n) {
         const int c = cnt[i];
         for (int k = 0; k < c; ++k)
-            live += nbr[sidx(k, (uint32_t)i, np)] >= 0;
+            live += gidx[slot_of(chunk0[i], stride[i], k)] >= 0;
     }
     part[threadIdx.x] = live;
     __syncthreads();
@@ -615,9 +746,13 @@ struct pd_sim {
     int device = 0;
     int integrator = PD_EULER, layout = PD_LAYOUT_SLOT;
     double dt = 0.0, damping = 0.0;
-    int n = 0, np = 0, K = 0, kc = 0;
+    int n = 0, K = 0;
     int tension_only = 0;
-    int lpn = 1, block = 256;
+    int lpn = 4, tile_nodes = 128;
+    int hmax = 0;           // halo capacity of a tile (LDS budget)
+    int hstride = 0;        // largest halo of the current tiles
+    size_t lds_budget = 0;  // bytes of LDS a workgroup may use
+    int ntiles = 0;
     bool has_buffers = false, has_corr = false, has_btypes = false,
          has_cnode = false;
     int nregimes = 1, nbond_types = 1;
@@ -626,25 +761,29 @@ struct pd_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    std::vector<double> h_corr;  // (N,K) as given to build
-    std::vector<int> h_btypes;   // (N,K)
+    std::vector<double> h_coords;  // (N,3) for tiling
+    std::vector<double> h_corr;    // (N,K) as given to build
+    std::vector<int> h_btypes;     // (N,K)
 
     DevBuf<NodeRec> rec[2];
+    DevBuf<TileInfo> tiles;
     DevBuf<double> r0, vol, u, ud, udd, force, body_force, bcv, fbcv, dens,
         damage, cnode, corr, k_tab, s_tab, pc_tab;
-    DevBuf<int> flags, family, cnt, nbr, nlist_ref, n_neigh, out_nlist,
-        out_nneigh;
-    DevBuf<unsigned short> pos, state;
+    DevBuf<int> flags, family, cnt, tnode, halo, gidx, stride, nlist_ref,
+        n_neigh, out_nlist, out_nneigh;
+    DevBuf<long long> chunk0;
+    DevBuf<unsigned short> loc, pos, state;
     DevBuf<unsigned long long> counter;
 };
 
 namespace {
 
-void run_parallel(int n, const std::function<void(int, int)> &fn)
+void run_parallel(int n, const std::function<void(int, int)> &fn,
+                  int min_per_thread = 4096)
 {
     const unsigned hw = std::thread::hardware_concurrency();
     int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
-    if (n < 4096)
+    if (n < min_per_thread)
         nt = 1;
     std::vector<std::thread> th;
     const int chunk = (n + nt - 1) / nt;
@@ -682,12 +821,21 @@ int dm_of(const pd_sim *s)
     return s->has_cnode ? DM_FIELD : DM_PMB;
 }
 
+size_t lds_bytes(const pd_sim *s)
+{
+    const bool nlin = s->nregimes > 1 || s->nbond_types > 1;
+    (void)s;
+    return LDS_BYTES;
+}
+
 StepArgs make_args(pd_sim *s)
 {
     StepArgs a;
     a.n = s->n;
-    a.np = s->np;
     a.K = s->K;
+    a.hstride = s->hstride;
+    a.ntiles = s->ntiles;
+    a.tiles_per_xcd = (s->ntiles + XCDS - 1) / XCDS;
     a.integrator = s->integrator;
     a.layout = s->layout;
     a.has_corr = s->has_corr ? 1 : 0;
@@ -701,6 +849,9 @@ StepArgs make_args(pd_sim *s)
     a.k4 = s->k * 4.0;
     a.rec_old = s->rec[s->cur].p;
     a.rec_new = s->rec[1 - s->cur].p;
+    a.tiles = s->tiles.p;
+    a.tnode = s->tnode.p;
+    a.halo = s->halo.p;
     a.u = s->u.p;
     a.ud = s->ud.p;
     a.udd = s->udd.p;
@@ -711,45 +862,63 @@ StepArgs make_args(pd_sim *s)
     a.fbcv = s->fbcv.p;
     a.dens = s->dens.p;
     a.cnt = s->cnt.p;
-    a.nbr = s->nbr.p;
+    a.loc = s->loc.p;
+    a.gidx = s->gidx.p;
     a.corr = s->corr.p;
     a.state = s->state.p;
     a.k_tab = s->k_tab.p;
     a.s_tab = s->s_tab.p;
     a.pc_tab = s->pc_tab.p;
+    a.chunk0 = s->chunk0.p;
+    a.stride = s->stride.p;
     a.nlist_ref = s->nlist_ref.p;
     a.n_neigh = s->n_neigh.p;
     return a;
 }
 
+template <int DM, int LPN>
+hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
+{
+    const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
+    const unsigned block = (unsigned)(s->tile_nodes * LPN);
+    const size_t shmem = lds_bytes(s);
+    static thread_local int configured_device = -1;
+    static thread_local size_t configured_bytes = 0;
+    if (configured_device != s->device || configured_bytes < shmem) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_step<DM, LPN>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess)
+            return e;
+        configured_device = s->device;
+        configured_bytes = shmem;
+    }
+    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block), shmem,
+                       s->stream, a, dmag, fmag);
+    return hipGetLastError();
+}
+
 template <int DM>
 hipError_t launch_dm(pd_sim *s, const StepArgs &a, double dmag, double fmag)
 {
-    const unsigned threads = (unsigned)s->n * (unsigned)s->lpn;
-    const unsigned grid = (threads + s->block - 1) / s->block;
     switch (s->lpn) {
     case 1:
-        hipLaunchKernelGGL((k_step<DM, 1>), dim3(grid), dim3(s->block), 0,
-                           s->stream, a, dmag, fmag);
-        break;
+        return launch_one<DM, 1>(s, a, dmag, fmag);
     case 2:
-        hipLaunchKernelGGL((k_step<DM, 2>), dim3(grid), dim3(s->block), 0,
-                           s->stream, a, dmag, fmag);
-        break;
+        return launch_one<DM, 2>(s, a, dmag, fmag);
     case 4:
-        hipLaunchKernelGGL((k_step<DM, 4>), dim3(grid), dim3(s->block), 0,
-                           s->stream, a, dmag, fmag);
-        break;
+        return launch_one<DM, 4>(s, a, dmag, fmag);
     default:
-        hipLaunchKernelGGL((k_step<DM, 8>), dim3(grid), dim3(s->block), 0,
-                           s->stream, a, dmag, fmag);
-        break;
+        return launch_one<DM, 8>(s, a, dmag, fmag);
     }
-    return hipGetLastError();
 }
 
 int launch_step(pd_sim *s, double dmag, double fmag)
 {
+    if (s->ntiles == 0) {
+        s->cur = 1 - s->cur;
+        return PD_OK;
+    }
     const StepArgs a = make_args(s);
     hipError_t e;
     switch (dm_of(s)) {
@@ -782,6 +951,55 @@ int refresh_records(pd_sim *s)
     HIPCHK(hipStreamSynchronize(s->stream));
     s->cur = 0;
     return PD_OK;
+}
+
+uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Node order for tiling: Morton order of the initial coordinates on a grid
+// whose cell holds about one node.
+std::vector<int> morton_order(int n, const double *x)
+{
+    double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], x[3 * (size_t)i + d]);
+            hi[d] = std::max(hi[d], x[3 * (size_t)i + d]);
+        }
+    double vol = 1.0;
+    int dims = 0;
+    for (int d = 0; d < 3; ++d)
+        if (hi[d] > lo[d]) {
+            vol *= hi[d] - lo[d];
+            ++dims;
+        }
+    double h = dims ? std::pow(vol / n, 1.0 / dims) : 1.0;
+    if (!(h > 0.0))
+        h = 1.0;
+    std::vector<uint64_t> code(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t c = 0;
+        for (int d = 0; d < 3; ++d) {
+            double v = std::floor((x[3 * (size_t)i + d] - lo[d]) / h);
+            v = std::min(std::max(v, 0.0), 2097151.0);
+            c |= spread3((uint64_t)v) << d;
+        }
+        code[i] = c;
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return code[a] < code[b]; });
+    return order;
 }
 
 }  // namespace
@@ -855,6 +1073,10 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
         return fail(PD_E_ARG, "device index out of range");
 
     HIPCHK(hipSetDevice(device));
+    int lds_max = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_max,
+                                 hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                 device));
     pd_sim *s = new pd_sim();
     s->device = device;
     s->integrator = integrator;
@@ -862,9 +1084,14 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
     s->dt = dt;
     s->damping = damping;
     s->n = nnodes;
-    s->np = (nnodes + 63) & ~63;
     s->K = max_neighbours;
+    s->lds_budget = (size_t)lds_max;
+    if ((size_t)lds_max < LDS_BYTES)
+        return fail(PD_E_NODEV, "device has less than 80 KiB of LDS per "
+                                "workgroup");
+    s->hmax = LDS_CAP_PMB;
     const size_t N = (size_t)nnodes;
+    s->h_coords.assign(coords, coords + 3 * N);
 
     std::vector<int> flags(N);
     for (size_t i = 0; i < N; ++i) {
@@ -956,7 +1183,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
             nbond_types > 255)
             return fail(PD_E_LIMIT, "damage model table too large");
     }
-    const int N = s->n, K = s->K, NP = s->np;
+    const int N = s->n, K = s->K;
     if (nlin) {
         for (size_t q = 0; q < (size_t)N * K; ++q) {
             if (nlist[q] == -1)
@@ -970,8 +1197,9 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         }
     }
 
-    // compute rows: live entries, sorted ascending by j
+    // bond rows: live entries (j, reference column), sorted ascending by j
     std::vector<int> cnt(N);
+    std::vector<long long> rstart(N + 1, 0);
     std::atomic<int> bad{0};
     run_parallel(N, [&](int lo, int hi) {
         for (int i = lo; i < hi; ++i) {
@@ -1002,19 +1230,10 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         return fail(PD_E_ARG, "n_neigh out of range [0, max_neighbours]");
     if (bad == 2)
         return fail(PD_E_ARG, "nlist entry out of range [0, nnodes)");
-    int kc = 0;
     for (int i = 0; i < N; ++i)
-        kc = std::max(kc, cnt[i]);
-    s->kc = kc;
-    const size_t S = ((size_t)((kc + 3) / 4) + PAD_CHUNKS) * NP * 4;
-    std::vector<int> nbr(S, (int)DEAD);
-    std::vector<unsigned short> pos(S, 0);
-    std::vector<double> corr;
-    std::vector<unsigned short> state;
-    if (s->has_corr)
-        corr.assign(S, 0.0);
-    if (nlin)
-        state.assign(S, 0);
+        rstart[i + 1] = rstart[i] + cnt[i];
+    std::vector<int> rj(rstart[N]);
+    std::vector<unsigned short> rp(rstart[N]);
     run_parallel(N, [&](int lo, int hi) {
         std::vector<std::pair<int, int>> ent;
         for (int i = lo; i < hi; ++i) {
@@ -1033,27 +1252,165 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                                 const std::pair<int, int> &y) {
                                  return x.first < y.first;
                              });
-            for (int k = 0; k < (int)ent.size(); ++k) {
-                const size_t sl = sidx(k, (uint32_t)i, NP);
-                const size_t src = (size_t)i * K + ent[k].second;
-                nbr[sl] = ent[k].first;
-                pos[sl] = (unsigned short)ent[k].second;
-                if (s->has_corr)
-                    corr[sl] = s->h_corr[src];
-                if (nlin)
-                    state[sl] = (unsigned short)((s->h_btypes[src] << 8) |
-                                                 (regimes ? regimes[src] : 0));
+            for (size_t k = 0; k < ent.size(); ++k) {
+                rj[rstart[i] + k] = ent[k].first;
+                rp[rstart[i] + k] = (unsigned short)ent[k].second;
             }
         }
     });
+
+    // tiles: greedy over the Morton order, capped by tile_nodes and by the
+    // LDS halo capacity
+    s->hmax = nlin ? LDS_CAP_NLIN : LDS_CAP_PMB;
+    const std::vector<int> order = morton_order(N, s->h_coords.data());
+    std::vector<int> stamp(N, -1);
+    std::vector<TileInfo> tiles;
+    std::vector<int> tnode;
+    tnode.reserve(N);
+    {
+        int cur_nodes = 0, cur_halo = 0, tid = 0;
+        auto count_new = [&](int i, int t) {
+            int c = stamp[i] != t;
+            for (long long q = rstart[i]; q < rstart[i + 1]; ++q)
+                c += stamp[rj[q]] != t;
+            return c;
+        };
+        for (int q = 0; q < N; ++q) {
+            const int i = order[q];
+            if (1 + cnt[i] > s->hmax)
+                return fail(PD_E_LIMIT, "a node's neighbourhood exceeds the "
+                                        "LDS halo capacity");
+            int add = count_new(i, tid);
+            if (cur_nodes > 0 && (cur_nodes == s->tile_nodes ||
+                                  cur_halo + add > s->hmax)) {
+                TileInfo t;
+                t.node_off = (int)tnode.size() - cur_nodes;
+                t.nodes = cur_nodes;
+                t.halo = cur_halo;
+                tiles.push_back(t);
+                ++tid;
+                cur_nodes = 0;
+                cur_halo = 0;
+                add = count_new(i, tid);
+            }
+            stamp[i] = tid;
+            for (long long r = rstart[i]; r < rstart[i + 1]; ++r)
+                stamp[rj[r]] = tid;
+            cur_halo += add;
+            ++cur_nodes;
+            tnode.push_back(i);
+        }
+        if (cur_nodes > 0) {
+            TileInfo t;
+            t.node_off = (int)tnode.size() - cur_nodes;
+            t.nodes = cur_nodes;
+            t.halo = cur_halo;
+            tiles.push_back(t);
+        }
+    }
+    // within a tile, nodes by descending bond count: the nodes sharing a
+    // wavefront then run loops of nearly equal length
+    for (const auto &t : tiles)
+        std::stable_sort(tnode.begin() + t.node_off,
+                         tnode.begin() + t.node_off + t.nodes,
+                         [&](int x, int y) { return cnt[x] > cnt[y]; });
+    const int ntiles = (int)tiles.size();
+    long long halo_total = 0, chunk_total = 0;
+    for (auto &t : tiles) {
+        t.halo_off = (int)halo_total;
+        halo_total += t.halo;
+        int nch = 0;
+        for (int l = 0; l < t.nodes; ++l)
+            nch = std::max(nch, (cnt[tnode[t.node_off + l]] + 3) >> 2);
+        t.nch = nch;
+        t.pad = 0;
+        t.chunk_off = chunk_total;
+        chunk_total += (long long)(nch + PAD_CHUNKS) * t.nodes;
+    }
+    if (halo_total > 0x7fffffffLL)
+        return fail(PD_E_LIMIT, "total halo size exceeds 2^31");
+    const size_t S = (size_t)chunk_total * 4;
+    std::vector<int> halo(halo_total);
+    std::vector<unsigned short> loc(S, LOC_DEAD);
+    std::vector<int> gidx(S, (int)DEAD);
+    std::vector<unsigned short> pos(S, 0);
+    std::vector<double> corr;
+    std::vector<unsigned short> state;
+    if (s->has_corr)
+        corr.assign(S, 0.0);
+    if (nlin)
+        state.assign(S, 0);
+    std::vector<long long> chunk0(N);
+    std::vector<int> stride(N);
+    std::atomic<int> halo_mismatch{0};
+    run_parallel(ntiles, [&](int lo, int hi) {
+        std::vector<int> lmap(N, -1);
+        std::vector<int> hl;
+        for (int t = lo; t < hi; ++t) {
+            const TileInfo &ti = tiles[t];
+            hl.clear();
+            for (int l = 0; l < ti.nodes; ++l) {
+                const int i = tnode[ti.node_off + l];
+                hl.push_back(i);
+                for (long long r = rstart[i]; r < rstart[i + 1]; ++r)
+                    hl.push_back(rj[r]);
+            }
+            std::sort(hl.begin(), hl.end());
+            hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+            if ((int)hl.size() != ti.halo) {
+                halo_mismatch = 1;
+                continue;
+            }
+            for (size_t h = 0; h < hl.size(); ++h) {
+                lmap[hl[h]] = (int)h;
+                halo[ti.halo_off + h] = hl[h];
+            }
+            for (int l = 0; l < ti.nodes; ++l) {
+                const int i = tnode[ti.node_off + l];
+                chunk0[i] = ti.chunk_off + l;
+                stride[i] = ti.nodes;
+                for (int k = 0; k < cnt[i]; ++k) {
+                    const int j = rj[rstart[i] + k];
+                    const int p = rp[rstart[i] + k];
+                    const size_t sl = slot_of(ti.chunk_off + l, ti.nodes, k);
+                    loc[sl] = (unsigned short)lmap[j];
+                    gidx[sl] = j;
+                    pos[sl] = (unsigned short)p;
+                    const size_t src = (size_t)i * K + p;
+                    if (s->has_corr)
+                        corr[sl] = s->h_corr[src];
+                    if (nlin)
+                        state[sl] = (unsigned short)(
+                            (s->h_btypes[src] << 8) | (regimes ? regimes[src] : 0));
+                }
+            }
+            for (int h : hl)
+                lmap[h] = -1;
+        }
+    }, 16);
+    if (halo_mismatch)
+        return fail(PD_E_STATE, "internal: tile halo count mismatch");
+    for (const auto &t : tiles)
+        if (t.halo > s->hmax)
+            return fail(PD_E_LIMIT, "internal: tile halo over capacity");
 
     HIPCHK(hipStreamSynchronize(s->stream));
     auto chk = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && rc == PD_OK)
             rc = fail(PD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
+    s->ntiles = ntiles;
+    s->hstride = 1;
+    for (const auto &t : tiles)
+        s->hstride = std::max(s->hstride, t.halo);
+    chk(upload(s->tiles, tiles.data(), tiles.size()), "tiles");
+    chk(upload(s->tnode, tnode.data(), tnode.size()), "tnode");
+    chk(upload(s->halo, halo.data(), halo.size()), "halo");
     chk(upload(s->cnt, cnt.data(), (size_t)N), "cnt");
-    chk(upload(s->nbr, nbr.data(), nbr.size()), "nbr");
+    chk(upload(s->chunk0, chunk0.data(), (size_t)N), "chunk0");
+    chk(upload(s->stride, stride.data(), (size_t)N), "stride");
+    chk(upload(s->loc, loc.data(), loc.size()), "loc");
+    chk(upload(s->gidx, gidx.data(), gidx.size()), "gidx");
     chk(upload(s->pos, pos.data(), pos.size()), "pos");
     if (s->has_corr)
         chk(upload(s->corr, corr.data(), corr.size()), "corr");
@@ -1141,7 +1498,7 @@ int pd_set_break_mode(pd_sim *s, int tension_only)
     return PD_OK;
 }
 
-int pd_set_launch(pd_sim *s, int lanes_per_node, int block_size)
+int pd_set_launch(pd_sim *s, int lanes_per_node, int tile_nodes)
 {
     int rc = check_sim(s);
     if (rc)
@@ -1149,10 +1506,15 @@ int pd_set_launch(pd_sim *s, int lanes_per_node, int block_size)
     if (lanes_per_node != 1 && lanes_per_node != 2 && lanes_per_node != 4 &&
         lanes_per_node != 8)
         return fail(PD_E_ARG, "lanes_per_node must be 1, 2, 4 or 8");
-    if (block_size < 64 || block_size > 256 || block_size % 64)
-        return fail(PD_E_ARG, "block_size must be 64, 128, 192 or 256");
+    if (tile_nodes < 64 || tile_nodes % 64 ||
+        tile_nodes * lanes_per_node > 1024)
+        return fail(PD_E_ARG, "tile_nodes must be a multiple of 64 with "
+                              "tile_nodes * lanes_per_node <= 1024");
+    if (s->has_buffers && tile_nodes != s->tile_nodes)
+        return fail(PD_E_STATE, "tile_nodes must be set before "
+                                "pd_create_buffers");
     s->lpn = lanes_per_node;
-    s->block = block_size;
+    s->tile_nodes = tile_nodes;
     return PD_OK;
 }
 
@@ -1214,9 +1576,9 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
     const int N = s->n;
     const int blocks = (N + 255) / 256;
     hipLaunchKernelGGL(k_export, dim3(blocks), dim3(256), 0, s->stream, N,
-                       s->np, s->K, s->layout, s->cnt.p, s->nbr.p, s->pos.p,
-                       s->family.p, s->n_neigh.p, s->out_nlist.p,
-                       s->out_nneigh.p, s->damage.p);
+                       s->K, s->layout, s->cnt.p, s->chunk0.p, s->stride.p,
+                       s->gidx.p, s->pos.p, s->family.p, s->n_neigh.p,
+                       s->out_nlist.p, s->out_nneigh.p, s->damage.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     const size_t B = 3 * (size_t)N * sizeof(double);
@@ -1266,8 +1628,8 @@ int pd_get_regimes(pd_sim *s, int *regimes)
     HIPCHK(hipMemsetAsync(out.p, 0, NK * sizeof(int), s->stream));
     const int blocks = (s->n + 255) / 256;
     hipLaunchKernelGGL(k_export_regimes, dim3(blocks), dim3(256), 0,
-                       s->stream, s->n, s->np, s->K, s->cnt.p, s->pos.p,
-                       s->state.p, out.p);
+                       s->stream, s->n, s->K, s->cnt.p, s->chunk0.p,
+                       s->stride.p, s->pos.p, s->state.p, out.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(regimes, out.p, NK * sizeof(int), hipMemcpyDeviceToHost));
@@ -1287,7 +1649,8 @@ int pd_live_bonds(pd_sim *s, long long *count)
                           s->stream));
     const int blocks = (s->n + 255) / 256;
     hipLaunchKernelGGL(k_live, dim3(blocks), dim3(256), 0, s->stream, s->n,
-                       s->np, s->cnt.p, s->nbr.p, s->counter.p);
+                       s->cnt.p, s->chunk0.p, s->stride.p, s->gidx.p,
+                       s->counter.p);
     HIPCHK(hipGetLastError());
     unsigned long long h = 0;
     HIPCHK(hipMemcpyAsync(&h, s->counter.p, sizeof(h), hipMemcpyDeviceToHost,
@@ -1303,6 +1666,25 @@ int pd_synchronize(pd_sim *s)
     if (rc)
         return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    return PD_OK;
+}
+
+int pd_tile_stats(pd_sim *s, int *ntiles, int *hmax, int *hstride,
+                  long long *halo_total, long long *slots)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (ntiles)
+        *ntiles = s->ntiles;
+    if (hmax)
+        *hmax = s->hmax;
+    if (hstride)
+        *hstride = s->hstride;
+    if (halo_total)
+        *halo_total = (long long)s->halo.n;
+    if (slots)
+        *slots = (long long)s->loc.n;
     return PD_OK;
 }
 

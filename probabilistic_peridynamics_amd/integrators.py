@@ -279,11 +279,24 @@ class Integrator(ABC):
         self._need_sim()
         check(_lib.lib().pd_set_break_mode(self._sim, int(bool(tension_only))))
 
-    def set_launch(self, lanes_per_node=1, block_size=256):
-        """Lanes per node and threads per block of the step kernel."""
+    def set_launch(self, lanes_per_node=4, tile_nodes=128):
+        """Lanes per node and nodes per tile of the step kernel.
+
+        ``tile_nodes`` takes effect at the next ``create_buffers``."""
         self._need_sim()
         check(_lib.lib().pd_set_launch(self._sim, int(lanes_per_node),
-                                       int(block_size)))
+                                       int(tile_nodes)))
+
+    def tile_stats(self):
+        """dict(ntiles, hmax, hstride, halo_total, slots) of the tiling."""
+        self._need_sim()
+        nt, hm, hs = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        ht, sl = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        check(_lib.lib().pd_tile_stats(self._sim, ctypes.byref(nt),
+                                       ctypes.byref(hm), ctypes.byref(hs),
+                                       ctypes.byref(ht), ctypes.byref(sl)))
+        return dict(ntiles=nt.value, hmax=hm.value, hstride=hs.value,
+                    halo_total=ht.value, slots=sl.value)
 
     # -- internals -----------------------------------------------------
     @staticmethod

@@ -343,12 +343,14 @@ def test_stiffness_corrections_and_node_field():
     assert pc.rel_l2(got["force"], ref.force) < FORCE_RTOL
 
 
-@pytest.mark.parametrize("lpn,block", [(1, 64), (2, 128), (4, 256), (8, 192)])
-def test_launch_shapes_agree(lpn, block):
-    """Every lanes-per-node / block shape gives the oracle's result."""
+@pytest.mark.parametrize("lpn,tile", [(1, 256), (2, 128), (4, 256), (8, 64),
+                                      (4, 64)])
+def test_launch_shapes_agree(lpn, tile):
+    """Every lanes-per-node / tile size gives the oracle's result."""
     w = _bar("slot", pull=1e-5)
-    integ = configs.make_integrator(w)
-    integ.set_launch(lpn, block)
+    integ = configs.make_integrator(w, lanes_per_node=lpn, tile_nodes=tile)
+    st = integ.tile_stats()
+    assert st["ntiles"] >= w.nnodes // tile
     ref = oracle.SlotStepper(0, w.dt, w.coords, w.volume, w.family,
                              w.bc_types, w.bc_values, w.force_bc_types,
                              w.force_bc_values, w.nlist, w.bond_stiffness,
