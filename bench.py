@@ -16,8 +16,15 @@ Prints one JSON line:
                  with HIP events on the library's stream; peak 8 TB/s
   cpu_baseline   the oracle (the reference's Cython kernels restated in C,
                  one core) on a bounded sample of the same workload
+
+roofline.traffic (HBM bytes per launch) comes from the committed rocprofv3
+summary profiles/k_step__<workload>.txt (FETCH_SIZE and WRITE_SIZE passes,
+each scaled by the same counter's reading for a plain copy kernel, which
+`bench.py --copy-probe` runs); it is null when that summary was made from a
+different kernel source or launch shape than the one measured here.
 """
 import argparse
+import hashlib
 import json
 import os
 import pathlib
@@ -43,6 +50,50 @@ def algorithmic_bytes(w, live):
     if w.nregimes > 1 or w.nbond_types > 1:
         b += 8 * live
     return b
+
+
+def kernel_source_hash():
+    """sha1 (first 12 hex digits) of the step kernel's source file."""
+    src = ROOT / "probabilistic_peridynamics_amd" / "csrc" / "pd_hip.hip"
+    return hashlib.sha1(src.read_bytes()).hexdigest()[:12]
+
+
+def profile_traffic(workload, lpn, tile):
+    """HBM bytes per step-kernel launch from profiles/k_step__<workload>.txt
+    ("key: value" lines), or None if absent or made from another kernel
+    source / launch shape."""
+    f = ROOT / "profiles" / f"k_step__{workload}.txt"
+    if not f.exists():
+        return None
+    kv = {}
+    for line in f.read_text().splitlines():
+        if ":" in line and not line.startswith((" ", "#")):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        if (kv.get("kernel_source_sha1") != kernel_source_hash()
+                or int(kv.get("lanes_per_node", -1)) != lpn
+                or int(kv.get("tile_nodes", -1)) != tile):
+            return None
+        return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_probe(mib):
+    """A plain copy kernel of known size (read mib MiB, write mib MiB), five
+    launches, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8,
+                      "copy_probe_bytes_written": n * 8, "launches": 5}),
+          flush=True)
+    return 0
 
 
 def cpu_baseline(w, budget_s):
@@ -108,7 +159,11 @@ def main():
     ap.add_argument("--tile-nodes", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 to skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the counter-calibration copy kernel")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is a design (see the design notes), "
@@ -177,6 +232,7 @@ def main():
             "tiles": tiles["ntiles"],
             "halo_per_node": tiles["halo_total"] / w.nnodes,
             "device": f"{name} ({arch})",
+            "kernel_source_sha1": kernel_source_hash(),
         },
         "roofline": {
             "bound": "hbm",
@@ -184,7 +240,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profile_traffic(w.name, args.lanes_per_node,
+                                       args.tile_nodes),
             "algorithmic_bytes_per_launch": B,
             "launch_us": per_step_s * 1e6,
         },

@@ -24,13 +24,19 @@
 // Tiles are mapped to workgroups so that each XCD (blockIdx % 8) runs a
 // contiguous range of tiles, whose halos overlap in that XCD's L2.
 //
-// Data layout in HBM (N nodes, S bond slots):
+// Data layout in HBM (N nodes, S bond slots).  Every per-node array is kept
+// in tile order: internal node p is node tnode[p] of the caller's numbering,
+// a tile's nodes are a contiguous range of p, and a halo is a few contiguous
+// runs, so per-node loads and stores coalesce.  Inputs are permuted into
+// this order on upload and outputs back on download.
 //   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, V | rx, ry, rz, c) with
 //              r = r0 + u (ping-pong; the r0/V half is static in both) and
 //              c the per-node stiffness (or 0).
-//   tiles[T]   TileInfo: node range in tnode[], halo range in halo[], first
-//              chunk of the tile's bond slots, chunk rows.
-//   tnode      node ids in tile order; halo: ascending node ids per tile.
+//   tiles[T]   TileInfo: node range [node_off, node_off + nodes) of p, halo
+//              range in halo[], first chunk of the tile's bond slots, chunk
+//              rows.
+//   tnode[N]   caller's node id of internal node p; halo: ascending internal
+//              ids per tile.
 //   bond slots, in chunks of 4: chunk (c, l) of a tile with n nodes is
 //              chunk_off + c*n + l and holds bonds 4c..4c+3 of its l-th node,
 //              rows sorted ascending by j; a wavefront reads its nodes' c-th
@@ -41,14 +47,15 @@
 //                compact layout) -- read only on breaks and exports
 //     pos   u16  reference column of the slot (exports)
 //     corr  f64, state u16 (bond type << 8 | regime): optional
-//   chunk0[N], stride[N]: node i's chunk 0 and its tile's node count, so
-//              slot(i, k) = (chunk0 + (k/4)*stride)*4 + k%4
+//   chunk0[N], stride[N]: node p's chunk 0 and its tile's node count, so
+//              slot(p, k) = (chunk0 + (k/4)*stride)*4 + k%4
 //   u, ud, udd, force, body_force (N,3) f64 in the reference's layout
 //   flags[N]   bit d: displacement BC on DOF d; bit 3+d: force BC on DOF d
 //   bcv, fbcv, dens (N,3) f64, read only for flagged DOFs / dynamic
 //              integrators
 //   nlist_ref (N,K) i32 + n_neigh: the reference-layout list of the compact
-//              (Cython) layout, maintained by in-kernel compaction.
+//              (Cython) layout, in the caller's numbering, maintained by
+//              in-kernel compaction.
 //
 // Stretch arithmetic: exactly spatial.pyx's (dr = r_j - r_i,
 // l = sqrt((dr0^2 + dr1^2) + dr2^2), s = (l - l0)/l0), built without
@@ -316,7 +323,14 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
         const double2 p0 = *reinterpret_cast<const double2 *>(pe);
         const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
         const double2 p2 = *reinterpret_cast<const double2 *>(pe + 2 * PB);
-        const double2 p3 = *reinterpret_cast<const double2 *>(pe + 3 * PB);
+        // c_j is read only by the node-field model
+        double2 p3;
+        if (DM == DM_FIELD) {
+            p3 = *reinterpret_cast<const double2 *>(pe + 3 * PB);
+        } else {
+            p3.x = *reinterpret_cast<const double *>(pe + 3 * PB);
+            p3.y = 0.0;
+        }
         // spatial.pyx:20-31 with r = r0 + u: dr = r_j - r_i
         dx[q] = p2.x - ri.x;
         dy[q] = p2.y - ri.y;
@@ -438,6 +452,32 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     const TileInfo ti = a.tiles[tile];
     char *ldsb = reinterpret_cast<char *>(lds);
 
+    // node state and the first DEPTH chunks of halo indices are requested
+    // before the halo is staged, so their latency overlaps the staging; the
+    // bond loop then keeps DEPTH chunk loads in flight.  Chunk rows are
+    // clamped to the tile's longest row (a node's surplus loads are unused).
+    constexpr int DEPTH = 4;
+    const int l = threadIdx.x / LPN;
+    const int sub = threadIdx.x % LPN;
+    const bool valid = l < ti.nodes;
+    const ushort4 *L4 = reinterpret_cast<const ushort4 *>(a.loc);
+    const long long c0 = ti.chunk_off + (valid ? l : 0);
+    const int last_row = max(ti.nch, 1) - 1;
+    auto chunk_at = [&](int c) {
+        return L4[c0 + (long long)min(c, last_row) * ti.nodes];
+    };
+    ushort4 q0 = chunk_at(sub), q1 = chunk_at(sub + LPN),
+            q2 = chunk_at(sub + 2 * LPN), q3 = chunk_at(sub + 3 * LPN);
+    int cnt = 0;
+    const uint32_t i = (uint32_t)(ti.node_off + (valid ? l : 0));
+    double4 r0i = make_double4(0.0, 0.0, 0.0, 0.0), ri = r0i;
+    if (valid) {
+        const NodeRec *pi = a.rec_old + i;
+        r0i = pi->a;
+        ri = pi->b;
+        cnt = a.cnt[i];
+    }
+
     // stage the halo: piece x copies 16-byte piece x%4 of record x/4, SP
     // pieces per thread in flight at once (loads clamped in bounds, stores
     // predicated); the r0/V piece of DM_PMB becomes (r0z, 4 k V)
@@ -475,32 +515,19 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     }
     __syncthreads();
 
-    const int l = threadIdx.x / LPN;
-    const int sub = threadIdx.x % LPN;
-    const bool valid = l < ti.nodes;
-
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int nbroken = 0;
-    int cnt = 0;
-    uint32_t i = 0;
-    double4 r0i = make_double4(0.0, 0.0, 0.0, 0.0), ri = r0i;
     if (valid) {
-        i = (uint32_t)a.tnode[ti.node_off + l];
-        const NodeRec *pi = a.rec_old + i;
-        r0i = pi->a;
-        ri = pi->b;
-        cnt = a.cnt[i];
         const int nch = (cnt + 3) >> 2;
-        const ushort4 *L4 = reinterpret_cast<const ushort4 *>(a.loc);
-        const long long c0 = ti.chunk_off + l;
-        ushort4 cur = L4[c0 + (long long)sub * ti.nodes];
         for (int c = sub; c < nch; c += LPN) {
-            // prefetch the next chunk (in bounds: PAD_CHUNKS rows per tile)
-            const ushort4 nxt = L4[c0 + (long long)(c + LPN) * ti.nodes];
+            const ushort4 nx = chunk_at(c + DEPTH * LPN);
             const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
-            chunk4<DM>(a, ldsb, r0i, ri, cur, s0, ts, tk, tp, fx, fy, fz,
+            chunk4<DM>(a, ldsb, r0i, ri, q0, s0, ts, tk, tp, fx, fy, fz,
                        nbroken);
-            cur = nxt;
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = nx;
         }
     }
 
@@ -516,7 +543,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
 
     if (a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
         compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
-                    a.n_neigh, i, cnt);
+                    a.n_neigh, (uint32_t)a.tnode[i], cnt);
 
     // force boundary condition after the bond sum (peridynamics.pyx:93-97,
     // cl/peridynamics.cl:117-129)
@@ -634,25 +661,28 @@ __global__ void k_records(int n, const double *r0, const double *vol,
 
 // Live-slot count per node, the damage (family - n)/family
 // (peridynamics.pyx:29), and, for the slot layout, the reference-layout
-// neighbour list: -1 everywhere but at the live slots' columns.
-__global__ void k_export(int n, int K, int layout, const int *cnt,
-                         const long long *chunk0, const int *stride,
-                         const int *gidx, const unsigned short *pos,
-                         const int *family, const int *n_neigh_c,
-                         int *nlist_out, int *n_neigh_out, double *damage)
+// neighbour list: -1 everywhere but at the live slots' columns.  Thread p
+// handles internal node p and writes row tnode[p] of the outputs.
+__global__ void k_export(int n, int K, int layout, const int *tnode,
+                         const int *cnt, const long long *chunk0,
+                         const int *stride, const int *gidx,
+                         const unsigned short *pos, const int *family,
+                         const int *n_neigh_c, int *nlist_out,
+                         int *n_neigh_out, double *damage)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
         return;
+    const int i = tnode[p];
     int live;
     if (layout == PD_LAYOUT_SLOT) {
         int *row = nlist_out + (size_t)i * K;
-        for (int p = 0; p < K; ++p)
-            row[p] = -1;
+        for (int q = 0; q < K; ++q)
+            row[q] = -1;
         live = 0;
-        const int c = cnt[i];
+        const int c = cnt[p];
         for (int k = 0; k < c; ++k) {
-            const size_t s = slot_of(chunk0[i], stride[i], k);
+            const size_t s = slot_of(chunk0[p], stride[p], k);
             const int e = gidx[s];
             if (e >= 0) {
                 row[pos[s]] = e;
@@ -667,18 +697,20 @@ __global__ void k_export(int n, int K, int layout, const int *cnt,
     damage[i] = (f - (double)live) / f;
 }
 
-__global__ void k_export_regimes(int n, int K, const int *cnt,
-                                 const long long *chunk0, const int *stride,
+__global__ void k_export_regimes(int n, int K, const int *tnode,
+                                 const int *cnt, const long long *chunk0,
+                                 const int *stride,
                                  const unsigned short *pos,
                                  const unsigned short *state, int *out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
         return;
-    const int c = cnt[i];
+    const size_t row = (size_t)tnode[p] * K;
+    const int c = cnt[p];
     for (int k = 0; k < c; ++k) {
-        const size_t s = slot_of(chunk0[i], stride[i], k);
-        out[(size_t)i * K + pos[s]] = state[s] & 0xff;
+        const size_t s = slot_of(chunk0[p], stride[p], k);
+        out[row + pos[s]] = state[s] & 0xff;
     }
 }
 
@@ -761,9 +793,13 @@ struct pd_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    std::vector<double> h_coords;  // (N,3) for tiling
+    // build inputs in the caller's node order; uploaded in tile order by
+    // pd_create_buffers, which fixes that order
+    std::vector<double> h_coords, h_vol, h_bcv, h_fbcv, h_dens, h_cnode;
+    std::vector<int> h_flags;
     std::vector<double> h_corr;    // (N,K) as given to build
     std::vector<int> h_btypes;     // (N,K)
+    std::vector<int> perm;         // internal node p -> caller's node id
 
     DevBuf<NodeRec> rec[2];
     DevBuf<TileInfo> tiles;
@@ -804,6 +840,42 @@ hipError_t upload(DevBuf<T> &d, const T *h, size_t count)
     if (e != hipSuccess || count == 0)
         return e;
     return hipMemcpy(d.p, h, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// dst[p] = src[perm[p]] for rows of w values (caller order -> tile order)
+template <typename T>
+std::vector<T> to_internal(const std::vector<int> &perm, const T *src, int w)
+{
+    std::vector<T> dst(perm.size() * (size_t)w);
+    for (size_t p = 0; p < perm.size(); ++p)
+        for (int d = 0; d < w; ++d)
+            dst[p * w + d] = src[(size_t)perm[p] * w + d];
+    return dst;
+}
+
+// Upload rows of w values given in the caller's order into tile order.
+template <typename T>
+hipError_t upload_perm(DevBuf<T> &d, const std::vector<int> &perm,
+                       const T *src, int w)
+{
+    const std::vector<T> v = to_internal(perm, src, w);
+    return upload(d, v.data(), v.size());
+}
+
+// Download rows of w values in tile order into the caller's order.
+template <typename T>
+hipError_t download_perm(T *dst, const DevBuf<T> &d,
+                         const std::vector<int> &perm, int w)
+{
+    std::vector<T> v(perm.size() * (size_t)w);
+    hipError_t e = hipMemcpy(v.data(), d.p, v.size() * sizeof(T),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return e;
+    for (size_t p = 0; p < perm.size(); ++p)
+        for (int d2 = 0; d2 < w; ++d2)
+            dst[(size_t)perm[p] * w + d2] = v[p * w + d2];
+    return hipSuccess;
 }
 
 int check_sim(pd_sim *s)
@@ -1109,14 +1181,13 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
         if (e != hipSuccess && rc == PD_OK)
             rc = fail(PD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
-    chk(upload(s->r0, coords, 3 * N), "coords");
-    chk(upload(s->vol, volume, N), "volume");
-    chk(upload(s->flags, flags.data(), N), "flags");
-    chk(upload(s->family, family, N), "family");
-    chk(upload(s->bcv, bc_values, 3 * N), "bc_values");
-    chk(upload(s->fbcv, force_bc_values, 3 * N), "force_bc_values");
+    s->h_vol.assign(volume, volume + N);
+    s->h_flags = std::move(flags);
+    s->h_bcv.assign(bc_values, bc_values + 3 * N);
+    s->h_fbcv.assign(force_bc_values, force_bc_values + 3 * N);
     if (densities)
-        chk(upload(s->dens, densities, 3 * N), "densities");
+        s->h_dens.assign(densities, densities + 3 * N);
+    chk(upload(s->family, family, N), "family");
     chk(s->rec[0].alloc(N), "rec");
     chk(s->rec[1].alloc(N), "rec");
     chk(s->u.alloc(3 * N), "u");
@@ -1314,6 +1385,9 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         std::stable_sort(tnode.begin() + t.node_off,
                          tnode.begin() + t.node_off + t.nodes,
                          [&](int x, int y) { return cnt[x] > cnt[y]; });
+    std::vector<int> inv(N);  // caller's node id -> internal id
+    for (int q = 0; q < N; ++q)
+        inv[tnode[q]] = q;
     const int ntiles = (int)tiles.size();
     long long halo_total = 0, chunk_total = 0;
     for (auto &t : tiles) {
@@ -1344,16 +1418,16 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     std::vector<int> stride(N);
     std::atomic<int> halo_mismatch{0};
     run_parallel(ntiles, [&](int lo, int hi) {
-        std::vector<int> lmap(N, -1);
+        std::vector<int> lmap(N, -1);  // internal id -> halo position
         std::vector<int> hl;
         for (int t = lo; t < hi; ++t) {
             const TileInfo &ti = tiles[t];
             hl.clear();
             for (int l = 0; l < ti.nodes; ++l) {
                 const int i = tnode[ti.node_off + l];
-                hl.push_back(i);
+                hl.push_back(ti.node_off + l);
                 for (long long r = rstart[i]; r < rstart[i + 1]; ++r)
-                    hl.push_back(rj[r]);
+                    hl.push_back(inv[rj[r]]);
             }
             std::sort(hl.begin(), hl.end());
             hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
@@ -1367,13 +1441,13 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
             }
             for (int l = 0; l < ti.nodes; ++l) {
                 const int i = tnode[ti.node_off + l];
-                chunk0[i] = ti.chunk_off + l;
-                stride[i] = ti.nodes;
+                chunk0[ti.node_off + l] = ti.chunk_off + l;
+                stride[ti.node_off + l] = ti.nodes;
                 for (int k = 0; k < cnt[i]; ++k) {
                     const int j = rj[rstart[i] + k];
                     const int p = rp[rstart[i] + k];
                     const size_t sl = slot_of(ti.chunk_off + l, ti.nodes, k);
-                    loc[sl] = (unsigned short)lmap[j];
+                    loc[sl] = (unsigned short)lmap[inv[j]];
                     gidx[sl] = j;
                     pos[sl] = (unsigned short)p;
                     const size_t src = (size_t)i * K + p;
@@ -1406,7 +1480,19 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     chk(upload(s->tiles, tiles.data(), tiles.size()), "tiles");
     chk(upload(s->tnode, tnode.data(), tnode.size()), "tnode");
     chk(upload(s->halo, halo.data(), halo.size()), "halo");
-    chk(upload(s->cnt, cnt.data(), (size_t)N), "cnt");
+    s->perm = tnode;
+    chk(upload_perm(s->cnt, s->perm, cnt.data(), 1), "cnt");
+    chk(upload_perm(s->r0, s->perm, s->h_coords.data(), 3), "coords");
+    chk(upload_perm(s->vol, s->perm, s->h_vol.data(), 1), "volume");
+    chk(upload_perm(s->flags, s->perm, s->h_flags.data(), 1), "flags");
+    chk(upload_perm(s->bcv, s->perm, s->h_bcv.data(), 3), "bc_values");
+    chk(upload_perm(s->fbcv, s->perm, s->h_fbcv.data(), 3),
+        "force_bc_values");
+    if (!s->h_dens.empty())
+        chk(upload_perm(s->dens, s->perm, s->h_dens.data(), 3), "densities");
+    if (s->has_cnode)
+        chk(upload_perm(s->cnode, s->perm, s->h_cnode.data(), 1),
+            "node stiffness");
     chk(upload(s->chunk0, chunk0.data(), (size_t)N), "chunk0");
     chk(upload(s->stride, stride.data(), (size_t)N), "stride");
     chk(upload(s->loc, loc.data(), loc.size()), "loc");
@@ -1431,15 +1517,15 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         "nlist export");
     const size_t B = 3 * (size_t)N * sizeof(double);
     if (u)
-        chk(hipMemcpy(s->u.p, u, B, hipMemcpyHostToDevice), "u");
+        chk(upload_perm(s->u, s->perm, u, 3), "u");
     else
         chk(hipMemset(s->u.p, 0, B), "u");
     if (ud)
-        chk(hipMemcpy(s->ud.p, ud, B, hipMemcpyHostToDevice), "ud");
+        chk(upload_perm(s->ud, s->perm, ud, 3), "ud");
     else
         chk(hipMemset(s->ud.p, 0, B), "ud");
     if (udd)
-        chk(hipMemcpy(s->udd.p, udd, B, hipMemcpyHostToDevice), "udd");
+        chk(upload_perm(s->udd, s->perm, udd, 3), "udd");
     else
         chk(hipMemset(s->udd.p, 0, B), "udd");
     s->nregimes = nregimes;
@@ -1475,10 +1561,14 @@ int pd_set_node_stiffness(pd_sim *s, const double *c)
     HIPCHK(hipStreamSynchronize(s->stream));
     if (!c) {
         s->cnode.release();
+        s->h_cnode.clear();
         s->has_cnode = false;
     } else {
-        HIPCHK(upload(s->cnode, c, (size_t)s->n));
+        s->h_cnode.assign(c, c + s->n);
         s->has_cnode = true;
+        // uploaded in tile order, which pd_create_buffers fixes
+        if (s->has_buffers)
+            HIPCHK(upload_perm(s->cnode, s->perm, c, 1));
     }
     if (s->has_buffers) {
         // the records carry c: rewrite both from the device u
@@ -1576,23 +1666,22 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
     const int N = s->n;
     const int blocks = (N + 255) / 256;
     hipLaunchKernelGGL(k_export, dim3(blocks), dim3(256), 0, s->stream, N,
-                       s->K, s->layout, s->cnt.p, s->chunk0.p, s->stride.p,
-                       s->gidx.p, s->pos.p, s->family.p, s->n_neigh.p,
-                       s->out_nlist.p, s->out_nneigh.p, s->damage.p);
+                       s->K, s->layout, s->tnode.p, s->cnt.p, s->chunk0.p,
+                       s->stride.p, s->gidx.p, s->pos.p, s->family.p,
+                       s->n_neigh.p, s->out_nlist.p, s->out_nneigh.p,
+                       s->damage.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
-    const size_t B = 3 * (size_t)N * sizeof(double);
     if (u)
-        HIPCHK(hipMemcpy(u, s->u.p, B, hipMemcpyDeviceToHost));
+        HIPCHK(download_perm(u, s->u, s->perm, 3));
     if (ud)
-        HIPCHK(hipMemcpy(ud, s->ud.p, B, hipMemcpyDeviceToHost));
+        HIPCHK(download_perm(ud, s->ud, s->perm, 3));
     if (udd)
-        HIPCHK(hipMemcpy(udd, s->udd.p, B, hipMemcpyDeviceToHost));
+        HIPCHK(download_perm(udd, s->udd, s->perm, 3));
     if (force)
-        HIPCHK(hipMemcpy(force, s->force.p, B, hipMemcpyDeviceToHost));
+        HIPCHK(download_perm(force, s->force, s->perm, 3));
     if (body_force)
-        HIPCHK(hipMemcpy(body_force, s->body_force.p, B,
-                         hipMemcpyDeviceToHost));
+        HIPCHK(download_perm(body_force, s->body_force, s->perm, 3));
     if (damage)
         HIPCHK(hipMemcpy(damage, s->damage.p, (size_t)N * sizeof(double),
                          hipMemcpyDeviceToHost));
@@ -1628,8 +1717,8 @@ int pd_get_regimes(pd_sim *s, int *regimes)
     HIPCHK(hipMemsetAsync(out.p, 0, NK * sizeof(int), s->stream));
     const int blocks = (s->n + 255) / 256;
     hipLaunchKernelGGL(k_export_regimes, dim3(blocks), dim3(256), 0,
-                       s->stream, s->n, s->K, s->cnt.p, s->chunk0.p,
-                       s->stride.p, s->pos.p, s->state.p, out.p);
+                       s->stream, s->n, s->K, s->tnode.p, s->cnt.p,
+                       s->chunk0.p, s->stride.p, s->pos.p, s->state.p, out.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(regimes, out.p, NK * sizeof(int), hipMemcpyDeviceToHost));
