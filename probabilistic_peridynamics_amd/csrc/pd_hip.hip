@@ -30,8 +30,8 @@
 // runs, so per-node loads and stores coalesce.  Inputs are permuted into
 // this order on upload and outputs back on download.
 //   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, V | rx, ry, rz, c) with
-//              r = r0 + u (ping-pong; the r0/V half is static in both) and
-//              c the per-node stiffness (or 0).
+//              r = r0 + u and c the per-node stiffness (or 0); ping-pong,
+//              with r0, V and c static in both.
 //   tiles[T]   TileInfo: node range [node_off, node_off + nodes) of p, halo
 //              range in halo[], first chunk of the tile's bond slots, chunk
 //              rows.
@@ -538,62 +538,53 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         fz += __shfl_xor(fz, off, LPN);
         nbroken += __shfl_xor(nbroken, off, LPN);
     }
-    if (!valid || sub != 0)
+    if (!valid)
         return;
 
-    if (a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
+    if (sub == 0 && a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
         compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
                     a.n_neigh, (uint32_t)a.tnode[i], cnt);
 
-    // force boundary condition after the bond sum (peridynamics.pyx:93-97,
-    // cl/peridynamics.cl:117-129)
+    // the node's lanes share its three components (after the xor reduction
+    // every lane holds the sums), so the per-node loads and stores coalesce
     const int fl = a.flags[i];
     const size_t b3 = 3 * (size_t)i;
-    const double bf[3] = {fx, fy, fz};
-    double F[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        F[d] = (fl >> (3 + d)) & 1 ? bf[d] + fmag * a.fbcv[b3 + d] : bf[d];
-        a.body_force[b3 + d] = bf[d];
-        a.force[b3 + d] = F[d];
-    }
-
-    double u[3] = {a.u[b3], a.u[b3 + 1], a.u[b3 + 2]};
-    if (a.integrator == PD_EULER) {
-        // peridynamics.pyx:179-185, cl/euler.cl:21-23
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d] : u[d] + a.dt * F[d];
-    } else if (a.integrator == PD_EULER_CROMER) {
-        // cl/euler_cromer.cl:29-34
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double ud = a.ud[b3 + d];
-            const double uddi = (F[d] - a.damping * ud) / a.dens[b3 + d];
-            ud = ud + uddi * a.dt;
-            a.udd[b3 + d] = uddi;
-            a.ud[b3 + d] = ud;
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d] : u[d] + a.dt * ud;
-        }
-    } else {
-        // cl/velocity_verlet.cl:29-35
-        const double hdt = a.dt / 2;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double ud1 = a.ud[b3 + d] + hdt * a.udd[b3 + d];
-            const double udd1 = (F[d] - a.damping * ud1) / a.dens[b3 + d];
+    for (int d = sub; d < 3; d += LPN) {
+        const size_t k = b3 + d;
+        const double bf = d == 0 ? fx : (d == 1 ? fy : fz);
+        // force boundary condition after the bond sum
+        // (peridynamics.pyx:93-97, cl/peridynamics.cl:117-129)
+        const double F = (fl >> (3 + d)) & 1 ? bf + fmag * a.fbcv[k] : bf;
+        a.body_force[k] = bf;
+        a.force[k] = F;
+        const bool fixed = (fl >> d) & 1;
+        double u = a.u[k];
+        if (a.integrator == PD_EULER) {
+            // peridynamics.pyx:179-185, cl/euler.cl:21-23
+            u = fixed ? dmag * a.bcv[k] : u + a.dt * F;
+        } else if (a.integrator == PD_EULER_CROMER) {
+            // cl/euler_cromer.cl:29-34
+            double ud = a.ud[k];
+            const double udd = (F - a.damping * ud) / a.dens[k];
+            ud = ud + udd * a.dt;
+            a.udd[k] = udd;
+            a.ud[k] = ud;
+            u = fixed ? dmag * a.bcv[k] : u + a.dt * ud;
+        } else {
+            // cl/velocity_verlet.cl:29-35
+            const double hdt = a.dt / 2;
+            const double ud1 = a.ud[k] + hdt * a.udd[k];
+            const double udd1 = (F - a.damping * ud1) / a.dens[k];
             const double ud = ud1 + hdt * udd1;
-            a.ud[b3 + d] = ud;
-            a.udd[b3 + d] = udd1;
-            u[d] = (fl >> d) & 1 ? dmag * a.bcv[b3 + d]
-                                 : u[d] + a.dt * (ud + hdt * udd1);
+            a.ud[k] = ud;
+            a.udd[k] = udd1;
+            u = fixed ? dmag * a.bcv[k] : u + a.dt * (ud + hdt * udd1);
         }
+        a.u[k] = u;
+        // r = r0 + u for the next step; the record's c (w) is static
+        const double r0d = d == 0 ? r0i.x : (d == 1 ? r0i.y : r0i.z);
+        reinterpret_cast<double *>(&a.rec_new[i].b)[d] = r0d + u;
     }
-    a.u[b3] = u[0];
-    a.u[b3 + 1] = u[1];
-    a.u[b3 + 2] = u[2];
-    a.rec_new[i].b =
-        make_double4(r0i.x + u[0], r0i.y + u[1], r0i.z + u[2], ri.w);
 }
 
 // The stretch pipeline of bond<DM_PMB>, one bond per thread, for checking:
