@@ -1086,8 +1086,13 @@ int pd_device_name(int device, char *name, int name_len, char *arch,
 {
     hipDeviceProp_t p;
     HIPCHK(hipGetDeviceProperties(&p, device));
-    if (name && name_len > 0)
-        snprintf(name, name_len, "%s", p.name);
+    if (name && name_len > 0) {
+        if (p.name[0])
+            snprintf(name, name_len, "%s", p.name);
+        else  // the marketing name is not always populated
+            snprintf(name, name_len, "HIP device %d, %d CUs", device,
+                     p.multiProcessorCount);
+    }
     if (arch && arch_len > 0)
         snprintf(arch, arch_len, "%s", p.gcnArchName);
     return PD_OK;

@@ -272,3 +272,121 @@ def test_create_crack():
     assert np.array_equal(nl, [[3, 2, -1, 0], [2, -1, 0, 0], [0, 1, 0, 0],
                                [0, 0, 0, 0]])
     assert np.array_equal(n_neigh, [2, 1, 2, 1])
+
+
+# --- n-linear (bond_force3) state machine, pinned by closed forms ----------
+# No reference test calls bond_force3/4; these pin the restatement of
+# cl/peridynamics.cl:313-378 against the trilinear law it must reproduce.
+
+TRI_CS = np.array([0.002, 0.004, 0.008])
+TRI_K = np.array([1.0, 0.6, 0.1]) * 1.0e11
+
+
+def _plus_cs(k, cs):
+    """model.py:1042-1070 recurrence: c_0 = 0,
+    c_r = c_{r-1} + k_{r-1} cs_{r-1} - k_r cs_{r-1}."""
+    c = np.zeros_like(k)
+    for r in range(1, len(k)):
+        c[r] = c[r - 1] + k[r - 1] * cs[r - 1] - k[r] * cs[r - 1]
+    return c
+
+
+def _trilinear(s):
+    """The piecewise-linear bond force per unit volume, by geometry."""
+    k0, k1, k2 = TRI_K
+    s0, s1, _ = TRI_CS
+    if s < s0:
+        return k0 * s
+    if s < s1:
+        return k0 * s0 + k1 * (s - s0)
+    return k0 * s0 + k1 * (s1 - s0) + k2 * (s - s1)
+
+
+def _bond_stepper(regime=0, nregimes=3, k=TRI_K, cs=TRI_CS):
+    r0 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    nl = np.array([[1, -1], [0, -1]], dtype=np.int32)
+    z3 = np.zeros((2, 3))
+    zi = np.zeros((2, 3), np.int32)
+    return oracle.SlotStepper(
+        oracle.EULER, 0.0, r0, np.array([0.5, 0.25]),
+        np.array([1, 1], np.int32), zi, z3, zi, z3, nl,
+        np.array(k).reshape(1, -1), np.array(cs).reshape(1, -1),
+        plus_cs=_plus_cs(np.array(k), np.array(cs)).reshape(1, -1),
+        nregimes=nregimes, bond_types=np.zeros((2, 2), np.int32),
+        regimes=np.array([[regime, 0], [regime, 0]], np.int32))
+
+
+def _stretch_to(st, s):
+    """Move node 1 so that the bond's stretch is s; return the stretch the
+    Cython arithmetic computes from the rounded position."""
+    st.u[:] = 0.0
+    st.u[1, 0] = s
+    return ((1.0 + s) - 1.0) / 1.0
+
+
+def test_nlinear_force_follows_trilinear_law():
+    st = _bond_stepper()
+    for s in np.linspace(1e-4, 0.0079, 80):
+        sa = _stretch_to(st, s)
+        st(0.0, 0.0)
+        want = _trilinear(sa)
+        # node 0 is pulled towards node 1 with force density f(s) * V_1
+        assert st.force[0, 0] == pytest.approx(want * 0.25, rel=1e-12)
+        assert st.force[1, 0] == pytest.approx(-want * 0.5, rel=1e-12)
+        assert st.nlist[0, 0] == 1
+
+
+def test_nlinear_force_is_continuous_at_breakpoints():
+    for b in TRI_CS[:2]:
+        lo, hi = _bond_stepper(), _bond_stepper()
+        for st in (lo, hi):  # walk the regime up to just below b
+            for s in np.linspace(1e-4, b * (1 - 1e-6), 40):
+                _stretch_to(st, s)
+                st(0.0, 0.0)
+        _stretch_to(lo, b * (1 - 1e-9))
+        lo(0.0, 0.0)
+        _stretch_to(hi, b * (1 + 1e-9))
+        hi(0.0, 0.0)
+        assert hi.regimes[0, 0] == lo.regimes[0, 0] + 1
+        assert hi.force[0, 0] == pytest.approx(lo.force[0, 0], rel=1e-7)
+
+
+def test_nlinear_one_regime_change_per_step_and_break():
+    st = _bond_stepper()
+    _stretch_to(st, 0.006)            # regime 2 by stretch, but one step
+    st(0.0, 0.0)
+    assert st.regimes[0, 0] == 1
+    k1, c1 = TRI_K[1], _plus_cs(TRI_K, TRI_CS)[1]
+    sa = ((1.0 + 0.006) - 1.0)
+    assert st.force[0, 0] == pytest.approx((sa * k1 + c1) * 0.25, rel=1e-12)
+    st(0.0, 0.0)
+    assert st.regimes[0, 0] == 2
+    _stretch_to(st, 0.0085)           # past the last breakpoint: breaks
+    st(0.0, 0.0)
+    assert st.nlist[0, 0] == -1 and st.nlist[1, 0] == -1
+    assert np.all(st.force == 0.0)
+    # and unloading walks down one regime per step
+    down = _bond_stepper(regime=2)
+    _stretch_to(down, 0.001)
+    down(0.0, 0.0)
+    assert down.regimes[0, 0] == 1
+    down(0.0, 0.0)
+    assert down.regimes[0, 0] == 0
+
+
+def test_nlinear_single_regime_equals_pmb():
+    """nregimes = 1 tables give bond_force1: force s*k*V and a break at
+    s >= s0 (cl/peridynamics.cl:78)."""
+    k, s0 = 2.0e11, 0.005
+    for s, broken in ((0.003, False), (0.0049, False), (0.0051, True)):
+        tab = _bond_stepper(nregimes=1, k=[k], cs=[s0])
+        pmb = _bond_stepper(nregimes=1, k=[k], cs=[s0])
+        pmb = oracle.SlotStepper(
+            oracle.EULER, 0.0, pmb.coords, pmb.volume, pmb.family,
+            pmb.bc_types, pmb.bc_values, pmb.force_bc_types,
+            pmb.force_bc_values, pmb.nlist, k, s0, tension_only=True)
+        for st in (tab, pmb):
+            _stretch_to(st, s)
+            st(0.0, 0.0)
+        assert np.array_equal(tab.force, pmb.force)
+        assert (tab.nlist[0, 0] == -1) == broken == (pmb.nlist[0, 0] == -1)
