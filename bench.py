@@ -16,6 +16,12 @@ Prints one JSON line:
                  with HIP events on the library's stream; peak 8 TB/s
   cpu_baseline   the oracle (the reference's Cython kernels restated in C,
                  one core) on a bounded sample of the same workload
+  batched        the UQ batch (256 jittered bars of 5k-60k nodes, per-node
+                 stiffness fields, Euler dt 1e-5, 500 steps): sample-steps/s
+                 of the whole batch in one simulation, its steps captured in
+                 one hipGraph, against the same samples run one after another
+                 with plain per-step launches; launch shape from the
+                 autotuner run on the first 32 samples
 
 roofline.traffic (HBM bytes per launch) comes from the committed rocprofv3
 summary profiles/k_step__<workload>.txt (FETCH_SIZE and WRITE_SIZE passes,
@@ -96,6 +102,57 @@ def copy_probe(mib):
     return 0
 
 
+def uq_bench(nsamples, steps, tune_samples):
+    """Batched vs sequential sample-steps/s on the UQ batch."""
+    from probabilistic_peridynamics_amd import autotune, batch, configs
+    t0 = time.perf_counter()
+    ws = configs.uq_samples(nsamples)
+    nodes = sum(w.nnodes for w in ws)
+    log(f"[bench] uq: {len(ws)} samples, {nodes} nodes, built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    tb = batch.SampleBatch(ws[:tune_samples])
+    best, table = autotune.autotune_batch(tb, steps=10)
+    del tb
+    log(f"[bench] uq autotune: best {best} "
+        + " ".join(f"{c}:{ms:.2f}ms" for c, ms in table))
+    t0 = time.perf_counter()
+    b = batch.SampleBatch(ws, lanes_per_node=best[0], tile_nodes=best[1])
+    log(f"[bench] uq batch set up in {time.perf_counter() - t0:.1f}s, "
+        f"{b.integrator.tile_stats()['ntiles']} tiles")
+    ms_batched = b.run(steps, graph=True)
+    live_end = b.integrator.live_bonds()
+    del b
+    ms_seq = 0.0
+    for w in ws:
+        integ = batch.solo_integrator(w, lanes_per_node=best[0],
+                                      tile_nodes=best[1])
+        d, f = w.magnitudes(steps)
+        ms_seq += integ.timed_run(steps, d, f)
+        del integ
+    ss_b = len(ws) * steps / (ms_batched / 1e3)
+    ss_s = len(ws) * steps / (ms_seq / 1e3)
+    log(f"[bench] uq: batched {ms_batched:.1f} ms, sequential {ms_seq:.1f} "
+        f"ms for {steps} steps")
+    return {
+        "workload": f"uq_{len(ws)}_samples_{nodes}_nodes",
+        "samples": len(ws), "nodes": nodes, "steps": steps,
+        "integrator": "euler", "dt": ws[0].dt,
+        "batched_sample_steps_per_s": ss_b,
+        "sequential_sample_steps_per_s": ss_s,
+        "speedup": ss_b / ss_s,
+        "batched_ms": ms_batched, "sequential_ms": ms_seq,
+        "live_bonds_start": int(sum(w.n_neigh.sum() for w in ws)),
+        "live_bonds_end": live_end,
+        "launch_shape": {"lanes_per_node": best[0], "tile_nodes": best[1]},
+        "autotune": [[list(c), ms] for c, ms in table],
+        "graph": True,
+    }
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(w, budget_s):
     """Time the oracle on this workload: break_bonds + bond_force
     (peridynamics.pyx order, one thread) and the integrator's update, whole
@@ -161,6 +218,10 @@ def main():
                     help="CPU baseline budget (0 to skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the counter-calibration copy kernel")
+    ap.add_argument("--uq-samples", type=int, default=256,
+                    help="samples of the batched UQ comparison (0 to skip)")
+    ap.add_argument("--uq-steps", type=int, default=500)
+    ap.add_argument("--uq-tune-samples", type=int, default=32)
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
@@ -201,6 +262,11 @@ def main():
     achieved = B / per_step_s / 1e9
     print(f"[bench] {args.steps} steps: {ms:.3f} ms (events), {wall*1e3:.3f}"
           f" ms wall, live {live0}->{live1}", file=sys.stderr, flush=True)
+
+    del integ
+    uq = None
+    if args.uq_samples > 0:
+        uq = uq_bench(args.uq_samples, args.uq_steps, args.uq_tune_samples)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -246,6 +312,7 @@ def main():
             "launch_us": per_step_s * 1e6,
         },
         "cpu_baseline": cpu,
+        "batched": uq,
     }
     print(json.dumps(out), flush=True)
     return 0

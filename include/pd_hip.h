@@ -113,8 +113,9 @@ int pd_set_break_mode(pd_sim *sim, int tension_only);
 /* Launch shape of the step kernel: lanes per node (1, 2, 4 or 8) and the
  * largest number of nodes per tile (a multiple of 64, with
  * tile_nodes * lanes_per_node <= 1024; one workgroup runs one tile).
- * Defaults: 4 and 128.  lanes_per_node may change at any time; tile_nodes
- * must be set before pd_create_buffers, which builds the tiles.  A tile's
+ * Defaults: 4 and 128.  lanes_per_node takes effect at the next step;
+ * tile_nodes at the next pd_create_buffers, which builds the tiles (stepping
+ * in between fails with PD_E_STATE).  A tile's
  * halo (its nodes and all their neighbours) is capped at 1280 entries (1184
  * for n-linear models), so a node may have at most 1279 (1183) bonds. */
 int pd_set_launch(pd_sim *sim, int lanes_per_node, int tile_nodes);
@@ -124,6 +125,13 @@ int pd_set_launch(pd_sim *sim, int lanes_per_node, int tile_nodes);
  * LDS each workgroup requests), total halo entries, bond slots. */
 int pd_tile_stats(pd_sim *sim, int *ntiles, int *hmax, int *hstride,
                   long long *halo_total, long long *slots);
+
+/* Tiling groups: group (N) i32, or NULL for one group.  Tiles never span two
+ * groups, so the disjoint meshes of a batch of samples concatenated into one
+ * simulation are tiled one sample at a time.  No counterpart in the
+ * reference: PeriPy runs its probabilistic samples one simulate() call each
+ * (model.py:1523-1542).  Must be set before pd_create_buffers. */
+int pd_set_groups(pd_sim *sim, const int *group);
 
 /* Integrator.__call__: one step (break, bond force, boundary conditions,
  * integration).  Asynchronous with respect to the host. */
@@ -139,6 +147,13 @@ int pd_run(pd_sim *sim, int nsteps, const double *disp_mags,
  * stream every step kernel is launched on; *elapsed_ms is the time between
  * them (synchronous). */
 int pd_run_timed(pd_sim *sim, int nsteps, const double *disp_mags,
+                 const double *force_mags, float *elapsed_ms);
+
+/* pd_run captured as one hipGraph and launched once (synchronous); if
+ * elapsed_ms is not NULL it receives the graph's run time, measured with
+ * events on the library's stream.  Replaces the step loop of Model.simulate
+ * (model.py:1295-1301) when the magnitudes of all steps are known. */
+int pd_run_graph(pd_sim *sim, int nsteps, const double *disp_mags,
                  const double *force_mags, float *elapsed_ms);
 
 /* Integrator.write (integrators.py:285-300): damage, then device -> host

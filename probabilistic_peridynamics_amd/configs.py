@@ -25,6 +25,14 @@ Workloads (sizes as named):
                c_B = 2.5 c_A, interface 0.5 c_A with A's breakpoints;
                Euler-Cromer, damping 2.5e6, rho 2400, dt 1e-8, ends pulled
                apart in x by 1e-9 m per step.
+  uq_samples:  256 jittered 5:1:1 bars (jitter 0.25 dx, dx = 0.01) of
+               N ~ logU[5000, 60000] nodes (sizes from default_rng(3),
+               sample s from default_rng(1000 + s)); per-node stiffness
+               c_i = c exp(g_i), g a Gaussian random field on the lattice
+               (squared-exponential covariance, correlation length 5 dx,
+               sigma 0.3); s0 = 0.005; the bar_euler loading (x < 2 dx
+               clamped, x > L - 2 dx pulled +x by 1e-8 m per step);
+               Euler dt = 1e-5.
 """
 from dataclasses import dataclass, field
 
@@ -129,7 +137,7 @@ def remove_bonds(nlist, n_neigh, predicate, coords):
 
 
 def bar_euler(layout="slot", dt=1e-5, pull=1e-8):
-    """The 2560-node Euler bar."""
+    """The 2560-node Euler bar with a pre-crack."""
     dx = 0.01
     horizon = 3.01 * dx
     coords = lattice((40, 8, 8), dx)
@@ -169,7 +177,7 @@ def _bar_1m(shape, seed, origin=(0.0, 0.0, 0.0), layout="slot"):
 
 
 def beam_vv(shape=(470, 47, 47), seed=1, layout="slot"):
-    """The 1,038,230-node Velocity-Verlet beam (BASELINE configs[1])."""
+    """The 1,038,230-node Velocity-Verlet beam."""
     dx, horizon, coords, family, nlist, n_neigh = _bar_1m(shape, seed,
                                                           layout=layout)
     n = coords.shape[0]
@@ -208,7 +216,7 @@ def trilinear_tables(c_a):
 
 
 def composite_ec(shape=(470, 47, 47), seed=2):
-    """The 1M-node composite trilinear Euler-Cromer bar (configs[2])."""
+    """The 1M-node composite trilinear Euler-Cromer bar."""
     dx0 = 2.0 / shape[0]
     dx, horizon, coords, family, nlist, n_neigh = _bar_1m(
         shape, seed, origin=(0.0, -shape[1] * dx0 / 2, 0.0))
@@ -260,3 +268,66 @@ def make_integrator(w, context=None, lanes_per_node=None, tile_nodes=None):
                          w.critical_stretch, w.plus_cs, z, z, z, z, z,
                          np.zeros(n), None, w.nregimes, w.nbond_types)
     return integ
+
+
+def gaussian_random_field(shape, dx, corr_len, sigma, rng):
+    """Stationary Gaussian field on a lattice of ``shape`` cells with
+    covariance sigma^2 exp(-|d|^2 / (2 corr_len^2)), by circulant embedding:
+    the covariance is laid on a periodic grid padded by 4 correlation
+    lengths per axis (where it has decayed to exp(-8)), its FFT gives the
+    eigenvalues (negative round-off clipped), and white noise is coloured by
+    their square roots."""
+    pad = int(np.ceil(4.0 * corr_len / dx))
+    m = [n + pad for n in shape]
+    axes = []
+    for md in m:
+        k = np.arange(md)
+        axes.append(np.minimum(k, md - k) * dx)
+    d2 = (axes[0][:, None, None] ** 2 + axes[1][None, :, None] ** 2
+          + axes[2][None, None, :] ** 2)
+    cov = sigma ** 2 * np.exp(-d2 / (2.0 * corr_len ** 2))
+    lam = np.clip(np.fft.fftn(cov).real, 0.0, None)
+    w = rng.standard_normal(m)
+    z = np.fft.ifftn(np.sqrt(lam) * np.fft.fftn(w)).real
+    return z[:shape[0], :shape[1], :shape[2]]
+
+
+def uq_sizes(nsamples=256, seed=3, nmin=5000, nmax=60000):
+    """Bar shapes (5m, m, m) with 5 m^3 ~ N, N log-uniform in [nmin, nmax]."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsamples))
+    m = np.maximum(2, np.rint(np.cbrt(n / 5.0))).astype(int)
+    return [(5 * int(k), int(k), int(k)) for k in m]
+
+
+def uq_sample(shape, s, dx=0.01, corr=5.0, sigma=0.3):
+    """Sample s of the UQ batch on a (5m, m, m) bar (see module doc)."""
+    rng = np.random.default_rng(1000 + s)
+    horizon = 3.01 * dx
+    coords = lattice(shape, dx, jitter=0.25, seed=rng)
+    g = gaussian_random_field(shape, dx, corr * dx, sigma, rng).reshape(-1)
+    c = micromodulus(horizon)
+    family, nlist, n_neigh = neighbours(coords, horizon, "slot")
+    n = coords.shape[0]
+    length = shape[0] * dx
+    bc_types = np.zeros((n, 3), dtype=np.int32)
+    bc_values = np.zeros((n, 3))
+    clamp = coords[:, 0] < 2 * dx
+    bc_types[clamp] = 1
+    pulled = coords[:, 0] > length - 2 * dx
+    bc_types[pulled, 0] = 1
+    bc_values[pulled, 0] = 1.0
+    return Workload(
+        name=f"uq_sample_{s}_{n}", coords=coords, volume=np.full(n, dx ** 3),
+        family=family, nlist=nlist, n_neigh=n_neigh, horizon=horizon,
+        bond_stiffness=c, critical_stretch=0.005, dt=1e-5,
+        integrator="euler", bc_types=bc_types, bc_values=bc_values,
+        force_bc_types=np.zeros((n, 3), dtype=np.int32),
+        force_bc_values=np.zeros((n, 3)), disp_rate=1e-8,
+        extra={"dx": dx, "shape": shape, "cnode": c * np.exp(g), "g": g})
+
+
+def uq_samples(nsamples=256, seed=3, nmin=5000, nmax=60000):
+    """The batch of UQ samples (list of Workloads, see module doc)."""
+    return [uq_sample(shape, s)
+            for s, shape in enumerate(uq_sizes(nsamples, seed, nmin, nmax))]

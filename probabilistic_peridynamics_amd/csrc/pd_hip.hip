@@ -40,8 +40,9 @@
 //   bond slots, in chunks of 4: chunk (c, l) of a tile with n nodes is
 //              chunk_off + c*n + l and holds bonds 4c..4c+3 of its l-th node,
 //              rows sorted ascending by j; a wavefront reads its nodes' c-th
-//              chunks contiguously.  Each tile has 8 padding chunk rows so
-//              that the one-chunk-ahead prefetch stays in bounds.
+//              chunks contiguously.  A tile has as many chunk rows as its
+//              longest row needs (at least one); the kernel's look-ahead
+//              chunk loads are clamped to the last of them.
 //     loc   u16  halo-local index of j; 0xFFFF: broken or empty (hot loop)
 //     gidx  i32  j, with bit 31 DEAD and bit 30 NEW (broken in this step,
 //                compact layout) -- read only on breaks and exports
@@ -111,7 +112,6 @@ constexpr unsigned short LOC_DEAD = 0xFFFF;
 constexpr int MAX_NODES = 1 << 26;
 constexpr double GUARD = 1e-10;
 constexpr int MAX_TAB = 256;  // nbond_types * nregimes
-constexpr int PAD_CHUNKS = 8;
 constexpr int MAX_LPN = 8;
 constexpr int XCDS = 8;
 // Halo capacity of a tile: entries of 4 x 16 bytes, stored piece-major (piece
@@ -776,6 +776,7 @@ struct pd_sim {
     int hstride = 0;        // largest halo of the current tiles
     size_t lds_budget = 0;  // bytes of LDS a workgroup may use
     int ntiles = 0;
+    bool retile_pending = false;  // tile_nodes changed since the tiling
     bool has_buffers = false, has_corr = false, has_btypes = false,
          has_cnode = false;
     int nregimes = 1, nbond_types = 1;
@@ -791,6 +792,7 @@ struct pd_sim {
     std::vector<double> h_corr;    // (N,K) as given to build
     std::vector<int> h_btypes;     // (N,K)
     std::vector<int> perm;         // internal node p -> caller's node id
+    std::vector<int> h_group;      // tiling groups (empty: one group)
 
     DevBuf<NodeRec> rec[2];
     DevBuf<TileInfo> tiles;
@@ -939,11 +941,10 @@ StepArgs make_args(pd_sim *s)
     return a;
 }
 
+// Allow k_step<DM, LPN> its dynamic LDS (once per device and size).
 template <int DM, int LPN>
-hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
+hipError_t configure(pd_sim *s)
 {
-    const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
-    const unsigned block = (unsigned)(s->tile_nodes * LPN);
     const size_t shmem = lds_bytes(s);
     static thread_local int configured_device = -1;
     static thread_local size_t configured_bytes = 0;
@@ -956,8 +957,19 @@ hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
         configured_device = s->device;
         configured_bytes = shmem;
     }
-    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block), shmem,
-                       s->stream, a, dmag, fmag);
+    return hipSuccess;
+}
+
+template <int DM, int LPN>
+hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
+{
+    const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
+    const unsigned block = (unsigned)(s->tile_nodes * LPN);
+    hipError_t e = configure<DM, LPN>(s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block),
+                       lds_bytes(s), s->stream, a, dmag, fmag);
     return hipGetLastError();
 }
 
@@ -976,8 +988,48 @@ hipError_t launch_dm(pd_sim *s, const StepArgs &a, double dmag, double fmag)
     }
 }
 
+template <int DM>
+hipError_t configure_dm(pd_sim *s)
+{
+    switch (s->lpn) {
+    case 1:
+        return configure<DM, 1>(s);
+    case 2:
+        return configure<DM, 2>(s);
+    case 4:
+        return configure<DM, 4>(s);
+    default:
+        return configure<DM, 8>(s);
+    }
+}
+
+// Everything a step launch needs that is not a stream operation (so that
+// steps can be captured into a graph).
+int prepare_launch(pd_sim *s)
+{
+    hipError_t e;
+    switch (dm_of(s)) {
+    case DM_NLIN:
+        e = configure_dm<DM_NLIN>(s);
+        break;
+    case DM_FIELD:
+        e = configure_dm<DM_FIELD>(s);
+        break;
+    default:
+        e = configure_dm<DM_PMB>(s);
+        break;
+    }
+    if (e != hipSuccess)
+        return fail(PD_E_HIP, std::string("kernel attributes: ") +
+                                  hipGetErrorString(e));
+    return PD_OK;
+}
+
 int launch_step(pd_sim *s, double dmag, double fmag)
 {
+    if (s->retile_pending)
+        return fail(PD_E_STATE, "tile_nodes changed: call pd_create_buffers "
+                                "to re-tile before stepping");
     if (s->ntiles == 0) {
         s->cur = 1 - s->cur;
         return PD_OK;
@@ -1029,7 +1081,9 @@ uint64_t spread3(uint64_t v)
 
 // Node order for tiling: Morton order of the initial coordinates on a grid
 // whose cell holds about one node.
-std::vector<int> morton_order(int n, const double *x)
+// Node order for tiling: by group (when given), then by Morton code of the
+// coordinates on a grid of about one node per cell.
+std::vector<int> morton_order(int n, const double *x, const int *group)
 {
     double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
     for (int i = 0; i < n; ++i)
@@ -1060,8 +1114,14 @@ std::vector<int> morton_order(int n, const double *x)
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i)
         order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return code[a] < code[b]; });
+    if (group)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return group[a] != group[b] ? group[a] < group[b]
+                                        : code[a] < code[b];
+        });
+    else
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return code[a] < code[b]; });
     return order;
 }
 
@@ -1329,7 +1389,8 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     // tiles: greedy over the Morton order, capped by tile_nodes and by the
     // LDS halo capacity
     s->hmax = nlin ? LDS_CAP_NLIN : LDS_CAP_PMB;
-    const std::vector<int> order = morton_order(N, s->h_coords.data());
+    const std::vector<int> order = morton_order(
+        N, s->h_coords.data(), s->h_group.empty() ? nullptr : s->h_group.data());
     std::vector<int> stamp(N, -1);
     std::vector<TileInfo> tiles;
     std::vector<int> tnode;
@@ -1348,7 +1409,10 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                 return fail(PD_E_LIMIT, "a node's neighbourhood exceeds the "
                                         "LDS halo capacity");
             int add = count_new(i, tid);
-            if (cur_nodes > 0 && (cur_nodes == s->tile_nodes ||
+            const bool new_group =
+                cur_nodes > 0 && !s->h_group.empty() &&
+                s->h_group[i] != s->h_group[tnode.back()];
+            if (cur_nodes > 0 && (cur_nodes == s->tile_nodes || new_group ||
                                   cur_halo + add > s->hmax)) {
                 TileInfo t;
                 t.node_off = (int)tnode.size() - cur_nodes;
@@ -1395,7 +1459,8 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         t.nch = nch;
         t.pad = 0;
         t.chunk_off = chunk_total;
-        chunk_total += (long long)(nch + PAD_CHUNKS) * t.nodes;
+        // at least one row, which the kernel's clamped loads may read
+        chunk_total += (long long)std::max(nch, 1) * t.nodes;
     }
     if (halo_total > 0x7fffffffLL)
         return fail(PD_E_LIMIT, "total halo size exceeds 2^31");
@@ -1546,6 +1611,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     if (rc)
         return rc;
     s->has_buffers = true;
+    s->retile_pending = false;
     return PD_OK;
 }
 
@@ -1596,11 +1662,27 @@ int pd_set_launch(pd_sim *s, int lanes_per_node, int tile_nodes)
         tile_nodes * lanes_per_node > 1024)
         return fail(PD_E_ARG, "tile_nodes must be a multiple of 64 with "
                               "tile_nodes * lanes_per_node <= 1024");
+    // a new tile size takes effect when pd_create_buffers re-tiles; until
+    // then the current tiles (of the old size) cannot be stepped
     if (s->has_buffers && tile_nodes != s->tile_nodes)
-        return fail(PD_E_STATE, "tile_nodes must be set before "
-                                "pd_create_buffers");
+        s->retile_pending = true;
     s->lpn = lanes_per_node;
     s->tile_nodes = tile_nodes;
+    return PD_OK;
+}
+
+int pd_set_groups(pd_sim *s, const int *group)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (s->has_buffers)
+        return fail(PD_E_STATE, "groups must be set before pd_create_buffers");
+    if (!group) {
+        s->h_group.clear();
+        return PD_OK;
+    }
+    s->h_group.assign(group, group + s->n);
     return PD_OK;
 }
 
@@ -1648,6 +1730,64 @@ int pd_run_timed(pd_sim *s, int nsteps, const double *disp_mags,
     HIPCHK(hipEventRecord(s->ev1, s->stream));
     HIPCHK(hipEventSynchronize(s->ev1));
     HIPCHK(hipEventElapsedTime(elapsed_ms, s->ev0, s->ev1));
+    return PD_OK;
+}
+
+int pd_run_graph(pd_sim *s, int nsteps, const double *disp_mags,
+                 const double *force_mags, float *elapsed_ms)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!s->has_buffers)
+        return fail(PD_E_STATE, "pd_create_buffers has not been called");
+    if (nsteps < 0)
+        return fail(PD_E_ARG, "nsteps < 0");
+    if (nsteps == 0) {
+        if (elapsed_ms)
+            *elapsed_ms = 0.0f;
+        return PD_OK;
+    }
+    // the kernel's LDS attribute is set outside the capture
+    rc = prepare_launch(s);
+    if (rc)
+        return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int cur0 = s->cur;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int t = 0; t < nsteps && rc == PD_OK; ++t)
+        rc = launch_step(s, disp_mags ? disp_mags[t] : 0.0,
+                         force_mags ? force_mags[t] : 0.0);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s->stream, &graph);
+    if (rc != PD_OK || ec != hipSuccess) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        s->cur = cur0;
+        return rc != PD_OK ? rc
+                           : fail(PD_E_HIP, std::string("graph capture: ") +
+                                                hipGetErrorString(ec));
+    }
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess && elapsed_ms)
+        e = hipEventRecord(s->ev0, s->stream);
+    if (e == hipSuccess)
+        e = hipGraphLaunch(exec, s->stream);
+    if (e == hipSuccess && elapsed_ms)
+        e = hipEventRecord(s->ev1, s->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && elapsed_ms)
+        e = hipEventElapsedTime(elapsed_ms, s->ev0, s->ev1);
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        s->cur = cur0;
+        return fail(PD_E_HIP, std::string("graph run: ") +
+                                  hipGetErrorString(e));
+    }
     return PD_OK;
 }
 

@@ -250,6 +250,26 @@ class Integrator(ABC):
                                       dptr(f), ctypes.byref(ms)))
         return float(ms.value)
 
+    def run_graph(self, steps, displacement_bc_magnitudes=None,
+                  force_bc_magnitudes=None):
+        """:meth:`run` captured as one hipGraph and launched once; returns
+        the graph's run time in ms (HIP events on the library's stream)."""
+        self._need_sim()
+        d = self._mags(displacement_bc_magnitudes, steps)
+        f = self._mags(force_bc_magnitudes, steps)
+        ms = ctypes.c_float(0.0)
+        check(_lib.lib().pd_run_graph(self._sim, int(steps), dptr(d),
+                                      dptr(f), ctypes.byref(ms)))
+        return float(ms.value)
+
+    def set_groups(self, group):
+        """Per-node tiling groups (tiles never span two groups), or None.
+        Takes effect at the next ``create_buffers``."""
+        self._need_sim()
+        g = (None if group is None
+             else _lib.as_i32(group, (self.nnodes,), "group"))
+        check(_lib.lib().pd_set_groups(self._sim, iptr(g)))
+
     def synchronize(self):
         self._need_sim()
         check(_lib.lib().pd_synchronize(self._sim))
