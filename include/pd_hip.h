@@ -156,6 +156,27 @@ int pd_run_timed(pd_sim *sim, int nsteps, const double *disp_mags,
 int pd_run_graph(pd_sim *sim, int nsteps, const double *disp_mags,
                  const double *force_mags, float *elapsed_ms);
 
+/* Adjoint of the Euler step with a per-node stiffness field, for gradients
+ * of a loss of the final displacements with respect to the field.  No
+ * counterpart in the reference.  Usage: pd_adjoint_start, then nsteps steps
+ * by pd_step / pd_run (not pd_run_graph), then pd_adjoint_backward (any
+ * number of times).  The recording keeps the step at which each bond broke
+ * (the break mask is held fixed in the backward pass) and u every
+ * checkpoint_every steps; the backward pass replays each segment from its
+ * checkpoint with the step kernel (bitwise the recorded trajectory) and
+ * runs the adjoint gather kernel back through it.  Requires PD_EULER,
+ * PD_LAYOUT_SLOT and pd_set_node_stiffness.  pd_create_buffers or
+ * pd_adjoint_stop ends a recording. */
+int pd_adjoint_start(pd_sim *sim, int nsteps, int checkpoint_every);
+
+/* dL_du (N,3) f64: dL/du after the recorded steps.  dL_dc (N) f64 receives
+ * dL/dc for the node stiffness field c (k_ij = (c_i + c_j)/2).  Leaves the
+ * state as it was at the end of the recording. */
+int pd_adjoint_backward(pd_sim *sim, const double *dL_du, double *dL_dc);
+
+/* Free the recording. */
+int pd_adjoint_stop(pd_sim *sim);
+
 /* Integrator.write (integrators.py:285-300): damage, then device -> host
  * copies.  Any pointer may be NULL to skip that array.  nlist (N,K) and
  * n_neigh (N) come back in the layout given to pd_build, reflecting every

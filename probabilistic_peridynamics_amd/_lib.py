@@ -50,6 +50,9 @@ SIGNATURES = {
     "pd_set_groups": (ctypes.c_int, [_SIM, _I]),
     "pd_step": (ctypes.c_int, [_SIM, ctypes.c_double, ctypes.c_double]),
     "pd_run_graph": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D, _F]),
+    "pd_adjoint_start": (ctypes.c_int, [_SIM, ctypes.c_int, ctypes.c_int]),
+    "pd_adjoint_backward": (ctypes.c_int, [_SIM, _D, _D]),
+    "pd_adjoint_stop": (ctypes.c_int, [_SIM]),
     "pd_run": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D]),
     "pd_run_timed": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D, _F]),
     "pd_write": (ctypes.c_int, [_SIM, _D, _D, _D, _D, _D, _D, _I, _I]),

@@ -161,6 +161,8 @@ struct StepArgs {
     const long long *chunk0;
     const int *stride;
     int *nlist_ref, *n_neigh;
+    int *brk;  // if set: the step at which each slot's bond broke
+    int step;  // index of this step (for brk)
 };
 
 __host__ __device__ __forceinline__ size_t slot_of(long long chunk0,
@@ -396,6 +398,8 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
         if (DM != DM_NLIN) {
             if (!(sign_mask(sh[q], a.absmask) < a.s0h)) {
                 a.loc[s0 + q] = LOC_DEAD;
+                if (a.brk)
+                    a.brk[s0 + q] = a.step;
                 a.gidx[s0 + q] |=
                     (int)(DEAD | (a.layout == PD_LAYOUT_COMPACT ? NEWB : 0u));
                 ++nbroken;
@@ -421,6 +425,8 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
                 a.state[s0 + q] = (unsigned short)((t << 8) | nreg);
             if (nreg >= a.nregimes) {
                 a.loc[s0 + q] = LOC_DEAD;
+                if (a.brk)
+                    a.brk[s0 + q] = a.step;
                 a.gidx[s0 + q] |= (int)DEAD;
                 ++nbroken;
                 continue;
@@ -728,6 +734,139 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
         atomicAdd(total, part[0]);
 }
 
+// --- adjoint of the Euler step with a recorded break mask -------------------
+//
+// Forward (PD_EULER, node stiffness field c): u_{n+1} = u_n + dt F(u_n) on
+// free DOFs (prescribed DOFs are set), with
+//   F_m = sum_j k_mj V_j phi(d_mj),  phi(d) = (1/l0 - 1/l) d = (s/l) d,
+//   d_mj = r_j - r_m, k_mj = (c_m + c_j)/2,
+// over the bonds live in step n (broken at a step > n, from brk).  For a
+// loss L(u_N) and mu = P_free lambda_{n+1}:
+//   lambda_n = mu + dt J_n^T mu,
+//   (J^T mu)_m = sum_j k_mj A_mj (V_m mu_j - V_j mu_m),
+//   A = dphi/dd = (1/l0 - 1/l) I + d d^T / l^3  (symmetric, even in d),
+//   dL/dc_m += dt/2 sum_j (1/l0 - 1/l)_mj d_mj . (V_j mu_m - V_m mu_j).
+
+// Bond state at the start of step n0: alive iff it broke at a step >= n0.
+__global__ void k_adj_restore(int n, int n0, const int *cnt,
+                              const long long *chunk0, const int *stride,
+                              const int *brk, const unsigned short *loc0,
+                              unsigned short *loc, int *gidx)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int c = cnt[p];
+    for (int k = 0; k < c; ++k) {
+        const size_t s = slot_of(chunk0[p], stride[p], k);
+        const bool alive = brk[s] >= n0;
+        loc[s] = alive ? loc0[s] : LOC_DEAD;
+        const uint32_t e = (uint32_t)gidx[s] & ~(DEAD | NEWB);
+        gidx[s] = (int)(alive ? e : (e | DEAD));
+    }
+}
+
+// brk at the start of a recording: -1 for bonds already broken, else "never".
+__global__ void k_adj_init(int n, const int *cnt, const long long *chunk0,
+                           const int *stride, const unsigned short *loc,
+                           int *brk)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int c = cnt[p];
+    for (int k = 0; k < c; ++k) {
+        const size_t s = slot_of(chunk0[p], stride[p], k);
+        brk[s] = loc[s] == LOC_DEAD ? -1 : 0x7fffffff;
+    }
+}
+
+// Internal index of each slot's neighbour (inv: caller id -> internal id).
+__global__ void k_adj_jint(int n, const int *cnt, const long long *chunk0,
+                           const int *stride, const int *gidx,
+                           const int *inv, int *jint)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int c = cnt[p];
+    for (int k = 0; k < c; ++k) {
+        const size_t s = slot_of(chunk0[p], stride[p], k);
+        jint[s] = inv[(uint32_t)gidx[s] & KEYM];
+    }
+}
+
+// One backward step: ADJ_LPN lanes per node (bonds round-robin, xor-shuffle
+// reduction).  u: u_n; lam: lambda_{n+1}; writes lambda_n and adds this
+// step's term of dL/dc.  All arrays in internal node order.
+constexpr int ADJ_LPN = 4;
+__global__ __launch_bounds__(256) void k_adjoint(
+    int n, int step, double dt, const int *cnt, const long long *chunk0,
+    const int *stride, const int *brk, const int *jint, const int *flags,
+    const double *r0, const double *u, const double *vol, const double *c,
+    const double *lam, double *lam_out, double *gradc)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = t / ADJ_LPN, sub = t % ADJ_LPN;
+    const bool valid = m < n;
+    const int mm = valid ? m : 0;
+    const size_t b = 3 * (size_t)mm;
+    const int fm = flags[mm];
+    double mu_m[3], rm[3], r0m[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        mu_m[d] = (fm >> d) & 1 ? 0.0 : lam[b + d];
+        r0m[d] = r0[b + d];
+        rm[d] = r0m[d] + u[b + d];
+    }
+    const double vm = vol[mm], cm = c[mm];
+    double jt[3] = {0.0, 0.0, 0.0};
+    double gc = 0.0;
+    const int cn = valid ? cnt[mm] : 0;
+    for (int k = sub; k < cn; k += ADJ_LPN) {
+        const size_t s = slot_of(chunk0[mm], stride[mm], k);
+        if (brk[s] <= step)
+            continue;  // broken before or in this step: no force in step n
+        const int j = jint[s];
+        const size_t bj = 3 * (size_t)j;
+        const int fj = flags[j];
+        const double vj = vol[j];
+        double dd[3], d0[3], w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            dd[d] = (r0[bj + d] + u[bj + d]) - rm[d];
+            d0[d] = r0[bj + d] - r0m[d];
+            const double muj = (fj >> d) & 1 ? 0.0 : lam[bj + d];
+            w[d] = vm * muj - vj * mu_m[d];
+        }
+        const double l =
+            sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+        const double l0 =
+            sqrt((d0[0] * d0[0] + d0[1] * d0[1]) + d0[2] * d0[2]);
+        const double psi = 1.0 / l0 - 1.0 / l;
+        const double kk = (cm + c[j]) * 0.5;
+        const double dw = dd[0] * w[0] + dd[1] * w[1] + dd[2] * w[2];
+        const double il3 = 1.0 / (l * l * l);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            jt[d] += kk * (psi * w[d] + dd[d] * dw * il3);
+        gc -= 0.5 * psi * dw;
+    }
+#pragma unroll
+    for (int off = ADJ_LPN / 2; off > 0; off >>= 1) {
+        jt[0] += __shfl_xor(jt[0], off, ADJ_LPN);
+        jt[1] += __shfl_xor(jt[1], off, ADJ_LPN);
+        jt[2] += __shfl_xor(jt[2], off, ADJ_LPN);
+        gc += __shfl_xor(gc, off, ADJ_LPN);
+    }
+    if (!valid)
+        return;
+    for (int d = sub; d < 3; d += ADJ_LPN)
+        lam_out[b + d] = mu_m[d] + dt * jt[d];
+    if (sub == 0)
+        gradc[mm] += dt * gc;
+}
+
 }  // namespace pdk
 
 // ---------------------------------------------------------------------------
@@ -803,6 +942,14 @@ struct pd_sim {
     DevBuf<long long> chunk0;
     DevBuf<unsigned short> loc, pos, state;
     DevBuf<unsigned long long> counter;
+
+    // adjoint recording (Euler, node field): break steps, checkpoints
+    bool adj_recording = false, adj_ready = false, adj_replay = false;
+    int adj_nsteps = 0, adj_every = 0, step_counter = 0;
+    std::vector<double> adj_dmag, adj_fmag;
+    DevBuf<int> brk, jint;
+    DevBuf<unsigned short> loc0;
+    DevBuf<double> ckpt, seg, lam[2], gradc, fin_u, fin_force, fin_bforce;
 };
 
 namespace {
@@ -938,6 +1085,8 @@ StepArgs make_args(pd_sim *s)
     a.stride = s->stride.p;
     a.nlist_ref = s->nlist_ref.p;
     a.n_neigh = s->n_neigh.p;
+    a.brk = s->adj_recording ? s->brk.p : nullptr;
+    a.step = s->step_counter;
     return a;
 }
 
@@ -1025,6 +1174,51 @@ int prepare_launch(pd_sim *s)
     return PD_OK;
 }
 
+void adjoint_release(pd_sim *s)
+{
+    s->adj_recording = s->adj_ready = s->adj_replay = false;
+    s->adj_nsteps = s->adj_every = s->step_counter = 0;
+    s->adj_dmag.clear();
+    s->adj_fmag.clear();
+    s->brk.release();
+    s->jint.release();
+    s->loc0.release();
+    s->ckpt.release();
+    s->seg.release();
+    s->lam[0].release();
+    s->lam[1].release();
+    s->gradc.release();
+    s->fin_u.release();
+    s->fin_force.release();
+    s->fin_bforce.release();
+}
+
+// Bookkeeping of an adjoint recording after each forward step: the step's
+// magnitudes, and a checkpoint of u every adj_every steps.
+int after_step(pd_sim *s, double dmag, double fmag)
+{
+    if (s->adj_replay)
+        return PD_OK;
+    if (!s->adj_recording) {
+        s->adj_ready = false;  // stepped past a finished recording
+        return PD_OK;
+    }
+    s->adj_dmag.push_back(dmag);
+    s->adj_fmag.push_back(fmag);
+    ++s->step_counter;
+    const size_t B = 3 * (size_t)s->n;
+    if (s->step_counter % s->adj_every == 0 &&
+        s->step_counter < s->adj_nsteps)
+        HIPCHK(hipMemcpyAsync(s->ckpt.p + (s->step_counter / s->adj_every) * B,
+                              s->u.p, B * sizeof(double),
+                              hipMemcpyDeviceToDevice, s->stream));
+    if (s->step_counter == s->adj_nsteps) {
+        s->adj_recording = false;
+        s->adj_ready = true;
+    }
+    return PD_OK;
+}
+
 int launch_step(pd_sim *s, double dmag, double fmag)
 {
     if (s->retile_pending)
@@ -1051,7 +1245,7 @@ int launch_step(pd_sim *s, double dmag, double fmag)
         return fail(PD_E_HIP, std::string("step kernel launch: ") +
                                   hipGetErrorString(e));
     s->cur = 1 - s->cur;
-    return PD_OK;
+    return after_step(s, dmag, fmag);
 }
 
 // (Re)write both record buffers from r0, V, the device u and c.
@@ -1612,6 +1806,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         return rc;
     s->has_buffers = true;
     s->retile_pending = false;
+    adjoint_release(s);  // slot arrays were rebuilt
     return PD_OK;
 }
 
@@ -1748,6 +1943,9 @@ int pd_run_graph(pd_sim *s, int nsteps, const double *disp_mags,
             *elapsed_ms = 0.0f;
         return PD_OK;
     }
+    if (s->adj_recording)
+        return fail(PD_E_STATE, "an adjoint recording runs step by step "
+                                "(pd_run), not as a graph");
     // the kernel's LDS attribute is set outside the capture
     rc = prepare_launch(s);
     if (rc)
@@ -1788,6 +1986,169 @@ int pd_run_graph(pd_sim *s, int nsteps, const double *disp_mags,
         return fail(PD_E_HIP, std::string("graph run: ") +
                                   hipGetErrorString(e));
     }
+    return PD_OK;
+}
+
+int pd_adjoint_start(pd_sim *s, int nsteps, int checkpoint_every)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!s->has_buffers || s->retile_pending)
+        return fail(PD_E_STATE, "pd_create_buffers has not been called");
+    if (s->integrator != PD_EULER || s->layout != PD_LAYOUT_SLOT ||
+        !s->has_cnode || s->nregimes > 1 || s->nbond_types > 1)
+        return fail(PD_E_ARG, "the adjoint needs the Euler integrator, the "
+                              "slot layout and a node stiffness field");
+    if (nsteps < 1 || checkpoint_every < 1)
+        return fail(PD_E_ARG, "nsteps and checkpoint_every must be >= 1");
+    adjoint_release(s);
+    const size_t S = s->loc.n, N = (size_t)s->n, B = 3 * N;
+    const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    std::vector<int> inv(N);
+    for (size_t q = 0; q < N; ++q)
+        inv[s->perm[q]] = (int)q;
+    DevBuf<int> inv_d;
+    HIPCHK(upload(inv_d, inv.data(), N));
+    HIPCHK(s->brk.alloc(S));
+    HIPCHK(s->jint.alloc(S));
+    HIPCHK(s->loc0.alloc(S));
+    HIPCHK(s->ckpt.alloc((size_t)nck * B));
+    HIPCHK(s->seg.alloc((size_t)checkpoint_every * B));
+    HIPCHK(s->lam[0].alloc(B));
+    HIPCHK(s->lam[1].alloc(B));
+    HIPCHK(s->gradc.alloc(N));
+    HIPCHK(s->fin_u.alloc(B));
+    HIPCHK(s->fin_force.alloc(B));
+    HIPCHK(s->fin_bforce.alloc(B));
+    HIPCHK(hipMemcpyAsync(s->loc0.p, s->loc.p, S * sizeof(unsigned short),
+                          hipMemcpyDeviceToDevice, s->stream));
+    const int blocks = (s->n + 255) / 256;
+    hipLaunchKernelGGL(k_adj_init, dim3(blocks), dim3(256), 0, s->stream,
+                       s->n, s->cnt.p, s->chunk0.p, s->stride.p, s->loc.p,
+                       s->brk.p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_jint, dim3(blocks), dim3(256), 0, s->stream,
+                       s->n, s->cnt.p, s->chunk0.p, s->stride.p, s->gidx.p,
+                       inv_d.p, s->jint.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->ckpt.p, s->u.p, B * sizeof(double),
+                          hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->adj_nsteps = nsteps;
+    s->adj_every = checkpoint_every;
+    s->step_counter = 0;
+    s->adj_recording = true;
+    return PD_OK;
+}
+
+namespace {
+
+// Put the state at the start of step n0 from u (already in s->u): bond
+// states from brk, records from u.
+int adjoint_restore(pd_sim *s, int n0)
+{
+    const int blocks = (s->n + 255) / 256;
+    hipLaunchKernelGGL(k_adj_restore, dim3(blocks), dim3(256), 0, s->stream,
+                       s->n, n0, s->cnt.p, s->chunk0.p, s->stride.p,
+                       s->brk.p, s->loc0.p, s->loc.p, s->gidx.p);
+    HIPCHK(hipGetLastError());
+    return refresh_records(s);
+}
+
+}  // namespace
+
+int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!s->adj_ready)
+        return fail(PD_E_STATE, "no finished adjoint recording: call "
+                                "pd_adjoint_start and run its steps");
+    if (!dL_du || !dL_dc)
+        return fail(PD_E_ARG, "dL_du and dL_dc are required");
+    const size_t N = (size_t)s->n, B = 3 * N, BB = B * sizeof(double);
+    const int every = s->adj_every, nsteps = s->adj_nsteps;
+    const int nck = (nsteps + every - 1) / every;
+    hipStream_t st = s->stream;
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpyAsync(s->fin_u.p, s->u.p, BB, hipMemcpyDeviceToDevice,
+                          st));
+    HIPCHK(hipMemcpyAsync(s->fin_force.p, s->force.p, BB,
+                          hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->fin_bforce.p, s->body_force.p, BB,
+                          hipMemcpyDeviceToDevice, st));
+    {
+        const std::vector<double> v = to_internal(s->perm, dL_du, 3);
+        HIPCHK(hipMemcpyAsync(s->lam[0].p, v.data(), BB,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(s->gradc.p, 0, N * sizeof(double), st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    int cur = 0;
+    const int ablocks = (int)((N * ADJ_LPN + 255) / 256);
+    const double *cfield = s->cnode.p;
+    struct ReplayGuard {  // steps below are replays, also on early returns
+        pd_sim *s;
+        ~ReplayGuard() { s->adj_replay = false; }
+    } guard{s};
+    s->adj_replay = true;
+    for (int k = nck - 1; k >= 0 && rc == PD_OK; --k) {
+        const int n0 = k * every, n1 = std::min(n0 + every, nsteps);
+        HIPCHK(hipMemcpyAsync(s->u.p, s->ckpt.p + (size_t)k * B, BB,
+                              hipMemcpyDeviceToDevice, st));
+        rc = adjoint_restore(s, n0);
+        if (rc)
+            break;
+        HIPCHK(hipMemcpyAsync(s->seg.p, s->u.p, BB, hipMemcpyDeviceToDevice,
+                              st));
+        // replay the segment: the same kernel from the same state reproduces
+        // the recorded steps (and their breaks) bitwise
+        for (int n = n0; n < n1 - 1 && rc == PD_OK; ++n) {
+            s->step_counter = n;
+            rc = launch_step(s, s->adj_dmag[n], s->adj_fmag[n]);
+            if (rc == PD_OK)
+                HIPCHK(hipMemcpyAsync(s->seg.p + (size_t)(n - n0 + 1) * B,
+                                      s->u.p, BB, hipMemcpyDeviceToDevice,
+                                      st));
+        }
+        for (int n = n1 - 1; n >= n0 && rc == PD_OK; --n) {
+            hipLaunchKernelGGL(k_adjoint, dim3(ablocks), dim3(256), 0, st,
+                               s->n, n, s->dt, s->cnt.p, s->chunk0.p,
+                               s->stride.p, s->brk.p, s->jint.p, s->flags.p,
+                               s->r0.p, s->seg.p + (size_t)(n - n0) * B,
+                               s->vol.p, cfield, s->lam[cur].p,
+                               s->lam[1 - cur].p, s->gradc.p);
+            HIPCHK(hipGetLastError());
+            cur = 1 - cur;
+        }
+    }
+    s->adj_replay = false;
+    if (rc)
+        return rc;
+    // back to the end of the recording
+    HIPCHK(hipMemcpyAsync(s->u.p, s->fin_u.p, BB, hipMemcpyDeviceToDevice,
+                          st));
+    HIPCHK(hipMemcpyAsync(s->force.p, s->fin_force.p, BB,
+                          hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(s->body_force.p, s->fin_bforce.p, BB,
+                          hipMemcpyDeviceToDevice, st));
+    rc = adjoint_restore(s, nsteps);
+    if (rc)
+        return rc;
+    s->step_counter = nsteps;
+    HIPCHK(download_perm(dL_dc, s->gradc, s->perm, 1));
+    return PD_OK;
+}
+
+int pd_adjoint_stop(pd_sim *s)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    adjoint_release(s);
     return PD_OK;
 }
 

@@ -262,6 +262,28 @@ class Integrator(ABC):
                                       dptr(f), ctypes.byref(ms)))
         return float(ms.value)
 
+    def adjoint_start(self, steps, checkpoint_every=100):
+        """Record the next ``steps`` steps (run by ``__call__`` / ``run``)
+        for :meth:`adjoint_backward`: break steps and a checkpoint of u
+        every ``checkpoint_every`` steps.  Euler with a node stiffness
+        field only."""
+        self._need_sim()
+        check(_lib.lib().pd_adjoint_start(self._sim, int(steps),
+                                          int(checkpoint_every)))
+
+    def adjoint_backward(self, dL_du):
+        """dL/dc of the node stiffness field for dL/du (nnodes, 3) after the
+        recorded steps (the break mask of the recording held fixed)."""
+        self._need_sim()
+        g = _lib.as_f64(dL_du, (self.nnodes, 3), "dL_du")
+        out = np.zeros(self.nnodes)
+        check(_lib.lib().pd_adjoint_backward(self._sim, dptr(g), dptr(out)))
+        return out
+
+    def adjoint_stop(self):
+        self._need_sim()
+        check(_lib.lib().pd_adjoint_stop(self._sim))
+
     def set_groups(self, group):
         """Per-node tiling groups (tiles never span two groups), or None.
         Takes effect at the next ``create_buffers``."""

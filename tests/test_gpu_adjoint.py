@@ -1,0 +1,108 @@
+"""The GPU adjoint (pd_adjoint_*) against the torch reference gradient and
+central finite differences; checkpoint-interval independence; state kept."""
+import numpy as np
+import pytest
+
+import adjoint_ref
+import pdcases as pc
+from probabilistic_peridynamics_amd import _lib, adjoint
+from test_adjoint_ref import adjoint_workload
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    try:
+        ok = _lib.device_count() > 0
+    except OSError:
+        ok = False
+    if not ok:
+        pytest.fail("no HIP device: the gpu tests need an MI355X")
+
+
+def _setup(steps=40, every=16, shape=(16, 5, 5), seed=0):
+    w = adjoint_workload(shape=shape, seed=seed)
+    rng = np.random.default_rng(11)
+    sensors = rng.choice(w.nnodes, 24, replace=False)
+    target = rng.normal(scale=1e-6, size=(24, 3))
+    solver = adjoint.StiffnessAdjoint(w, steps, checkpoint_every=every)
+    return w, solver, sensors, target
+
+
+def _loss_grad(solver, g, sensors, target):
+    u = solver.forward(g)
+    dL = np.zeros_like(u)
+    dL[sensors] = 2.0 * (u[sensors] - target)
+    return float(((u[sensors] - target) ** 2).sum()), solver.backward(dL), u
+
+
+def test_adjoint_matches_reference_gradient():
+    w, solver, sensors, target = _setup()
+    g = w.extra["g"]
+    L, grad, u = _loss_grad(solver, g, sensors, target)
+    Lr, grad_r, ur, alive = adjoint_ref.loss_and_grad(
+        w, g, w.bond_stiffness, solver.steps, sensors, target)
+    # same trajectory and the same broken bonds
+    _, _, nl = solver.state()
+    ii, jj = adjoint_ref.edges(w.nlist)
+    cols = np.nonzero(w.nlist >= 0)[1]
+    np.testing.assert_array_equal(nl[ii, cols] >= 0, alive)
+    assert (~alive).sum() > 0
+    assert pc.rel_l2(u, ur) < 1e-12
+    assert L == pytest.approx(Lr, rel=1e-10)
+    assert pc.rel_l2(grad, grad_r) < 1e-10
+
+
+def test_adjoint_matches_central_differences():
+    w, solver, sensors, target = _setup()
+    g = w.extra["g"]
+    L, grad, _ = _loss_grad(solver, g, sensors, target)
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    worst = 0.0
+    for _ in range(8):
+        v = rng.standard_normal(w.nnodes)
+        Lp = _loss_grad(solver, g + h * v, sensors, target)[0]
+        Lm = _loss_grad(solver, g - h * v, sensors, target)[0]
+        fd = (Lp - Lm) / (2 * h)
+        worst = max(worst, abs(grad @ v - fd) / abs(fd))
+    assert worst < 1e-6
+
+
+def test_gradient_independent_of_checkpoint_interval():
+    grads = []
+    for every in (1, 7, 40, 100):
+        w, solver, sensors, target = _setup(every=every)
+        grads.append(_loss_grad(solver, w.extra["g"], sensors, target)[1])
+    for gr in grads[1:]:
+        np.testing.assert_array_equal(gr, grads[0])
+
+
+def test_backward_keeps_the_final_state_and_repeats():
+    w, solver, sensors, target = _setup()
+    g = w.extra["g"]
+    L, grad, u = _loss_grad(solver, g, sensors, target)
+    u1, d1, nl1 = solver.state()
+    dL = np.zeros_like(u)
+    dL[sensors] = 2.0 * (u[sensors] - target)
+    grad2 = solver.backward(dL)
+    u2, d2, nl2 = solver.state()
+    np.testing.assert_array_equal(grad, grad2)
+    np.testing.assert_array_equal(u1, u)
+    np.testing.assert_array_equal(u2, u)
+    np.testing.assert_array_equal(nl2, nl1)
+    np.testing.assert_array_equal(d2, d1)
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    w, solver, sensors, target = _setup()
+    g = torch.tensor(w.extra["g"], dtype=torch.float64, requires_grad=True)
+    L = adjoint.sensor_loss(g, solver, torch.as_tensor(sensors),
+                            torch.as_tensor(target))
+    L.backward()
+    Lr, grad_r, _, _ = adjoint_ref.loss_and_grad(
+        w, w.extra["g"], w.bond_stiffness, solver.steps, sensors, target)
+    assert float(L) == pytest.approx(Lr, rel=1e-10)
+    assert pc.rel_l2(g.grad.numpy(), grad_r) < 1e-10
