@@ -22,6 +22,9 @@ Prints one JSON line:
                  one hipGraph, against the same samples run one after another
                  with plain per-step launches; launch shape from the
                  autotuner run on the first 32 samples
+  composite      the 1M-node composite trilinear Euler-Cromer bar (three
+                 bond types, three regimes): step time, bond-updates/s and
+                 the HBM fraction with its algorithmic bytes
   adjoint        the stiffness-field adjoint on a 50,820-node jittered bar
                  (105 x 22 x 22, dx = 0.01, the 2560-node bar's loading):
                  2000 Euler steps, checkpoints every 100, loss = squared
@@ -157,6 +160,30 @@ def uq_bench(nsamples, steps, tune_samples):
     }
 
 
+def composite_bench(steps, warmup=3):
+    """Step time of the composite n-linear Euler-Cromer workload."""
+    from probabilistic_peridynamics_amd import configs
+    w = configs.composite_ec()
+    integ = configs.make_integrator(w)
+    dm, fm = w.magnitudes(warmup + steps)
+    integ.run(warmup, dm[:warmup], fm[:warmup])
+    integ.synchronize()
+    live0 = integ.live_bonds()
+    ms = integ.timed_run(steps, dm[warmup:], fm[warmup:])
+    live1 = integ.live_bonds()
+    per = ms / 1e3 / steps
+    live = 0.5 * (live0 + live1)
+    B = algorithmic_bytes(w, live)
+    log(f"[bench] composite: {per * 1e6:.1f} us/step, live {live0}->{live1}")
+    return {
+        "workload": w.name, "nodes": w.nnodes, "live_bonds": live,
+        "integrator": w.integrator, "steps": steps,
+        "ms_per_step": per * 1e3, "bond_updates_per_s": live / per,
+        "algorithmic_bytes_per_launch": B,
+        "hbm_frac": B / per / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def adjoint_bench(steps, every, ndirs, dt):
     """Forward/backward timing and FD agreement of the adjoint."""
     from probabilistic_peridynamics_amd import adjoint, configs
@@ -284,6 +311,8 @@ def main():
                     help="samples of the batched UQ comparison (0 to skip)")
     ap.add_argument("--uq-steps", type=int, default=500)
     ap.add_argument("--uq-tune-samples", type=int, default=32)
+    ap.add_argument("--composite-steps", type=int, default=50,
+                    help="steps of the composite workload (0 to skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint check (0 to skip)")
     ap.add_argument("--adjoint-dirs", type=int, default=32)
@@ -332,6 +361,10 @@ def main():
     uq = None
     if args.uq_samples > 0:
         uq = uq_bench(args.uq_samples, args.uq_steps, args.uq_tune_samples)
+
+    comp = None
+    if args.composite_steps > 0:
+        comp = composite_bench(args.composite_steps)
 
     adj = None
     if args.adjoint_steps > 0:
@@ -383,6 +416,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": uq,
+        "composite": comp,
         "adjoint": adj,
     }
     print(json.dumps(out), flush=True)

@@ -282,46 +282,32 @@ __device__ __noinline__ void compact_row(int *gidx, long long c0, int stride,
 // the step kernel
 // ---------------------------------------------------------------------------
 
-// A chunk of four bonds of node i.  e4 holds the neighbours' halo indices
-// (LOC_DEAD: broken or empty), s0 the slot index of the first bond.  The
-// pieces of halo entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, c), with
-// w = 4*k*V for DM_PMB and w = V otherwise.  The four
-// bonds' reads are issued first and their arithmetic is straight-line code
-// (a dead slot is computed on entry 0 and masked), so the four chains
-// interleave; only the exact-division path -- a stretch near a threshold,
-// or a square root outside the core's range -- and bond breaking branch.
 template <int DM>
 constexpr unsigned piece_bytes()
 {
     return 16u * (DM == DM_NLIN ? LDS_CAP_NLIN : LDS_CAP_PMB);
 }
 
-template <int DM>
-__device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
-                                       const double4 &r0i, const double4 &ri,
-                                       ushort4 e4, size_t s0,
-                                       const double *ts, const double *tk,
-                                       const double *tp, double &fx,
-                                       double &fy, double &fz, int &nbroken)
+// W bonds of node i at slots s0 .. s0+W-1: ev their halo indices (LOC_DEAD:
+// broken or empty), st their (bond type << 8 | regime) for DM_NLIN.
+template <int DM, int W>
+__device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
+                                      const double4 &r0i, const double4 &ri,
+                                      const unsigned *ev,
+                                      const unsigned short *st, size_t s0,
+                                      const double *ts, const double *tk,
+                                      const double *tp, double &fx,
+                                      double &fy, double &fz, int &nbroken)
 {
-    const unsigned ev[4] = {e4.x, e4.y, e4.z, e4.w};
-    unsigned short st[4] = {0, 0, 0, 0};
-    if (DM == DM_NLIN) {
-        const ushort4 s4 = *reinterpret_cast<const ushort4 *>(a.state + s0);
-        st[0] = s4.x;
-        st[1] = s4.y;
-        st[2] = s4.z;
-        st[3] = s4.w;
-    }
+    constexpr unsigned PB = piece_bytes<DM>();
     // per bond, what the force needs: dr, 0.5/l, s/2 and the weight
-    double dx[4], dy[4], dz[4], hl[4], sh[4], w[4];
-    bool live[4], slow[4];
+    double dx[W], dy[W], dz[W], hl[W], sh[W], w[W];
+    bool live[W], slow[W];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < W; ++q) {
         live[q] = ev[q] != LOC_DEAD;
         const unsigned idx = live[q] ? ev[q] : 0u;
         const char *pe = ldsb + 16u * idx;
-        constexpr unsigned PB = piece_bytes<DM>();
         const double2 p0 = *reinterpret_cast<const double2 *>(pe);
         const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
         const double2 p2 = *reinterpret_cast<const double2 *>(pe + 2 * PB);
@@ -363,14 +349,13 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
         slow[q] = live[q] & (!fast | near);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < W; ++q) {
         if (slow[q]) {
             // rare: near a threshold, or a square root outside the core's
             // range: IEEE square roots and division (spatial.pyx:87-97)
             const char *pe = ldsb + 16u * ev[q];
             const double2 p0 = *reinterpret_cast<const double2 *>(pe);
-            const double2 p1 =
-                *reinterpret_cast<const double2 *>(pe + piece_bytes<DM>());
+            const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
             const double qq = (dx[q] * dx[q] + dy[q] * dy[q]) + dz[q] * dz[q];
             const double ex = p0.x - r0i.x;
             const double ey = p0.y - r0i.y;
@@ -382,16 +367,12 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
             sh[q] = ((l - l0) / l0) * 0.5;
         }
     }
-    double cr[4] = {1.0, 1.0, 1.0, 1.0};
-    if (a.has_corr) {
-        const double4 c4 = *reinterpret_cast<const double4 *>(a.corr + s0);
-        cr[0] = c4.x;
-        cr[1] = c4.y;
-        cr[2] = c4.z;
-        cr[3] = c4.w;
-    }
+    double cr[W];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < W; ++q)
+        cr[q] = a.has_corr ? a.corr[s0 + q] : 1.0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
         if (!live[q])
             continue;
         double g;
@@ -442,6 +423,41 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
     }
 }
 
+// A chunk's bond data: the halo indices and, for DM_NLIN, the bond states.
+struct Chunk {
+    ushort4 loc, st;
+};
+
+// A chunk of four bonds of node i, slots s0 .. s0+3.  The pieces of halo
+// entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, c), with w = 4*k*V for
+// DM_PMB and w = V otherwise.  The scalar and field models take the four
+// bonds at once: their reads are issued first and the arithmetic is
+// straight-line code (a dead slot is computed on entry 0 and masked), so
+// the four chains interleave; the n-linear model, which holds more state
+// per bond, takes two at a time.  Only the exact-division path -- a stretch
+// near a threshold, or a square root outside the core's range -- and
+// breaking branch.
+template <int DM>
+__device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
+                                       const double4 &r0i, const double4 &ri,
+                                       const Chunk &ch, size_t s0,
+                                       const double *ts, const double *tk,
+                                       const double *tp, double &fx,
+                                       double &fy, double &fz, int &nbroken)
+{
+    const unsigned ev[4] = {ch.loc.x, ch.loc.y, ch.loc.z, ch.loc.w};
+    const unsigned short st[4] = {ch.st.x, ch.st.y, ch.st.z, ch.st.w};
+    if (DM != DM_NLIN) {
+        bonds<DM, 4>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
+                     nbroken);
+    } else {
+        bonds<DM, 2>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
+                     nbroken);
+        bonds<DM, 2>(a, ldsb, r0i, ri, ev + 2, st + 2, s0 + 2, ts, tk, tp,
+                     fx, fy, fz, nbroken);
+    }
+}
+
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
 // node taking its 4-bond chunks round-robin and reducing with cross-lane
 // shuffles.  LDS (dynamic, LDS_BYTES): the halo, four pieces of
@@ -469,11 +485,16 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     const ushort4 *L4 = reinterpret_cast<const ushort4 *>(a.loc);
     const long long c0 = ti.chunk_off + (valid ? l : 0);
     const int last_row = max(ti.nch, 1) - 1;
+    const ushort4 *S4 = reinterpret_cast<const ushort4 *>(a.state);
     auto chunk_at = [&](int c) {
-        return L4[c0 + (long long)min(c, last_row) * ti.nodes];
+        const long long k = c0 + (long long)min(c, last_row) * ti.nodes;
+        Chunk ch;
+        ch.loc = L4[k];
+        ch.st = DM == DM_NLIN ? S4[k] : make_ushort4(0, 0, 0, 0);
+        return ch;
     };
-    ushort4 q0 = chunk_at(sub), q1 = chunk_at(sub + LPN),
-            q2 = chunk_at(sub + 2 * LPN), q3 = chunk_at(sub + 3 * LPN);
+    Chunk q0 = chunk_at(sub), q1 = chunk_at(sub + LPN),
+          q2 = chunk_at(sub + 2 * LPN), q3 = chunk_at(sub + 3 * LPN);
     int cnt = 0;
     const uint32_t i = (uint32_t)(ti.node_off + (valid ? l : 0));
     double4 r0i = make_double4(0.0, 0.0, 0.0, 0.0), ri = r0i;
@@ -526,7 +547,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     if (valid) {
         const int nch = (cnt + 3) >> 2;
         for (int c = sub; c < nch; c += LPN) {
-            const ushort4 nx = chunk_at(c + DEPTH * LPN);
+            const Chunk nx = chunk_at(c + DEPTH * LPN);
             const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
             chunk4<DM>(a, ldsb, r0i, ri, q0, s0, ts, tk, tp, fx, fy, fz,
                        nbroken);
