@@ -43,7 +43,8 @@
 //              chunks contiguously.  A tile has as many chunk rows as its
 //              longest row needs (at least one); the kernel's look-ahead
 //              chunk loads are clamped to the last of them.
-//     loc   u16  halo-local index of j; 0xFFFF: broken or empty (hot loop)
+//     loc   u16  1 + halo-local index of j; 0: broken or empty (hot loop;
+//                entry 0 of a staged halo is a zero record)
 //     gidx  i32  j, with bit 31 DEAD and bit 30 NEW (broken in this step,
 //                compact layout) -- read only on breaks and exports
 //     pos   u16  reference column of the slot (exports)
@@ -108,7 +109,7 @@ namespace pdk {
 constexpr uint32_t DEAD = 0x80000000u;
 constexpr uint32_t NEWB = 0x40000000u;
 constexpr uint32_t KEYM = 0x3FFFFFFFu;
-constexpr unsigned short LOC_DEAD = 0xFFFF;
+constexpr unsigned short LOC_DEAD = 0;  // loc of a broken or empty slot
 constexpr int MAX_NODES = 1 << 26;
 constexpr double GUARD = 1e-10;
 constexpr int MAX_TAB = 256;  // nbond_types * nregimes
@@ -306,8 +307,8 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         live[q] = ev[q] != LOC_DEAD;
-        const unsigned idx = live[q] ? ev[q] : 0u;
-        const char *pe = ldsb + 16u * idx;
+        // a dead slot reads the zero record at entry 0 (weight 0)
+        const char *pe = ldsb + 16u * ev[q];
         const double2 p0 = *reinterpret_cast<const double2 *>(pe);
         const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
         const double2 p2 = *reinterpret_cast<const double2 *>(pe + 2 * PB);
@@ -460,13 +461,13 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
 
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
 // node taking its 4-bond chunks round-robin and reducing with cross-lane
-// shuffles.  LDS (dynamic, LDS_BYTES): the halo, four pieces of
+// shuffles.  LDS (static, LDS_BYTES): the halo, four pieces of
 // piece_bytes<DM>() each, then for DM_NLIN the three damage-model tables.
 template <int DM, int LPN>
 __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
                                                double fmag)
 {
-    extern __shared__ double2 lds[];
+    __shared__ double2 lds[LDS_BYTES / sizeof(double2)];
     const int b = blockIdx.x;
     const int tile = (b % XCDS) * a.tiles_per_xcd + b / XCDS;
     if (tile >= a.ntiles)
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     // stage the halo: piece x copies 16-byte piece x%4 of record x/4, SP
     // pieces per thread in flight at once (loads clamped in bounds, stores
     // predicated); the r0/V piece of DM_PMB becomes (r0z, 4 k V)
-    constexpr int SP = 6;
+    constexpr int SP = 10;
     constexpr unsigned PB = piece_bytes<DM>();
     const int np = 4 * ti.halo;
     for (int x0 = threadIdx.x; x0 < np; x0 += SP * blockDim.x) {
@@ -526,11 +527,14 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
                 double2 w = v[k];
                 if (DM == DM_PMB && (x & 3) == 1)
                     w.y = a.k4 * w.y;
-                *reinterpret_cast<double2 *>(ldsb + (x & 3) * PB +
-                                             16u * (unsigned)(x >> 2)) = w;
+                *reinterpret_cast<double2 *>(
+                    ldsb + (x & 3) * PB + 16u * (unsigned)((x >> 2) + 1)) = w;
             }
         }
     }
+    if (threadIdx.x < 4)  // entry 0: the zero record dead slots read
+        *reinterpret_cast<double2 *>(ldsb + threadIdx.x * PB) =
+            make_double2(0.0, 0.0);
     double *ts = reinterpret_cast<double *>(ldsb + 4 * PB);
     double *tk = ts + MAX_TAB, *tp = tk + MAX_TAB;
     if (DM == DM_NLIN) {
@@ -1054,13 +1058,6 @@ int dm_of(const pd_sim *s)
     return s->has_cnode ? DM_FIELD : DM_PMB;
 }
 
-size_t lds_bytes(const pd_sim *s)
-{
-    const bool nlin = s->nregimes > 1 || s->nbond_types > 1;
-    (void)s;
-    return LDS_BYTES;
-}
-
 StepArgs make_args(pd_sim *s)
 {
     StepArgs a;
@@ -1111,35 +1108,13 @@ StepArgs make_args(pd_sim *s)
     return a;
 }
 
-// Allow k_step<DM, LPN> its dynamic LDS (once per device and size).
-template <int DM, int LPN>
-hipError_t configure(pd_sim *s)
-{
-    const size_t shmem = lds_bytes(s);
-    static thread_local int configured_device = -1;
-    static thread_local size_t configured_bytes = 0;
-    if (configured_device != s->device || configured_bytes < shmem) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&k_step<DM, LPN>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-        if (e != hipSuccess)
-            return e;
-        configured_device = s->device;
-        configured_bytes = shmem;
-    }
-    return hipSuccess;
-}
-
 template <int DM, int LPN>
 hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
 {
     const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
     const unsigned block = (unsigned)(s->tile_nodes * LPN);
-    hipError_t e = configure<DM, LPN>(s);
-    if (e != hipSuccess)
-        return e;
-    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block),
-                       lds_bytes(s), s->stream, a, dmag, fmag);
+    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block), 0,
+                       s->stream, a, dmag, fmag);
     return hipGetLastError();
 }
 
@@ -1158,42 +1133,6 @@ hipError_t launch_dm(pd_sim *s, const StepArgs &a, double dmag, double fmag)
     }
 }
 
-template <int DM>
-hipError_t configure_dm(pd_sim *s)
-{
-    switch (s->lpn) {
-    case 1:
-        return configure<DM, 1>(s);
-    case 2:
-        return configure<DM, 2>(s);
-    case 4:
-        return configure<DM, 4>(s);
-    default:
-        return configure<DM, 8>(s);
-    }
-}
-
-// Everything a step launch needs that is not a stream operation (so that
-// steps can be captured into a graph).
-int prepare_launch(pd_sim *s)
-{
-    hipError_t e;
-    switch (dm_of(s)) {
-    case DM_NLIN:
-        e = configure_dm<DM_NLIN>(s);
-        break;
-    case DM_FIELD:
-        e = configure_dm<DM_FIELD>(s);
-        break;
-    default:
-        e = configure_dm<DM_PMB>(s);
-        break;
-    }
-    if (e != hipSuccess)
-        return fail(PD_E_HIP, std::string("kernel attributes: ") +
-                                  hipGetErrorString(e));
-    return PD_OK;
-}
 
 void adjoint_release(pd_sim *s)
 {
@@ -1432,7 +1371,7 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
     if ((size_t)lds_max < LDS_BYTES)
         return fail(PD_E_NODEV, "device has less than 80 KiB of LDS per "
                                 "workgroup");
-    s->hmax = LDS_CAP_PMB;
+    s->hmax = LDS_CAP_PMB - 1;
     const size_t N = (size_t)nnodes;
     s->h_coords.assign(coords, coords + 3 * N);
 
@@ -1603,7 +1542,8 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
 
     // tiles: greedy over the Morton order, capped by tile_nodes and by the
     // LDS halo capacity
-    s->hmax = nlin ? LDS_CAP_NLIN : LDS_CAP_PMB;
+    // entry 0 of a staged halo is the zero record of dead slots
+    s->hmax = (nlin ? LDS_CAP_NLIN : LDS_CAP_PMB) - 1;
     const std::vector<int> order = morton_order(
         N, s->h_coords.data(), s->h_group.empty() ? nullptr : s->h_group.data());
     std::vector<int> stamp(N, -1);
@@ -1723,7 +1663,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                     const int j = rj[rstart[i] + k];
                     const int p = rp[rstart[i] + k];
                     const size_t sl = slot_of(ti.chunk_off + l, ti.nodes, k);
-                    loc[sl] = (unsigned short)lmap[inv[j]];
+                    loc[sl] = (unsigned short)(lmap[inv[j]] + 1);
                     gidx[sl] = j;
                     pos[sl] = (unsigned short)p;
                     const size_t src = (size_t)i * K + p;
@@ -1967,10 +1907,6 @@ int pd_run_graph(pd_sim *s, int nsteps, const double *disp_mags,
     if (s->adj_recording)
         return fail(PD_E_STATE, "an adjoint recording runs step by step "
                                 "(pd_run), not as a graph");
-    // the kernel's LDS attribute is set outside the capture
-    rc = prepare_launch(s);
-    if (rc)
-        return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     const int cur0 = s->cur;
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
