@@ -431,11 +431,11 @@ struct Chunk {
 
 // A chunk of four bonds of node i, slots s0 .. s0+3.  The pieces of halo
 // entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, c), with w = 4*k*V for
-// DM_PMB and w = V otherwise.  The scalar and field models take the four
-// bonds at once: their reads are issued first and the arithmetic is
-// straight-line code (a dead slot is computed on entry 0 and masked), so
-// the four chains interleave; the n-linear model, which holds more state
-// per bond, takes two at a time.  Only the exact-division path -- a stretch
+// DM_PMB and w = V otherwise.  The bonds go two at a time: their reads are
+// issued first and the arithmetic is straight-line code (a dead slot
+// computes on the zero record and is masked), so the two chains interleave.
+// (Four at a time is no faster and needs 40 more registers, which the
+// field and n-linear models would spill.)  Only the exact-division path -- a stretch
 // near a threshold, or a square root outside the core's range -- and
 // breaking branch.
 template <int DM>
@@ -448,15 +448,10 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
 {
     const unsigned ev[4] = {ch.loc.x, ch.loc.y, ch.loc.z, ch.loc.w};
     const unsigned short st[4] = {ch.st.x, ch.st.y, ch.st.z, ch.st.w};
-    if (DM != DM_NLIN) {
-        bonds<DM, 4>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
-                     nbroken);
-    } else {
-        bonds<DM, 2>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
-                     nbroken);
-        bonds<DM, 2>(a, ldsb, r0i, ri, ev + 2, st + 2, s0 + 2, ts, tk, tp,
-                     fx, fy, fz, nbroken);
-    }
+    bonds<DM, 2>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
+                 nbroken);
+    bonds<DM, 2>(a, ldsb, r0i, ri, ev + 2, st + 2, s0 + 2, ts, tk, tp, fx, fy,
+                 fz, nbroken);
 }
 
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
@@ -504,6 +499,23 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         r0i = pi->a;
         ri = pi->b;
         cnt = a.cnt[i];
+    }
+    // with 4 or more lanes per node, lane d < 3 updates component d: its
+    // inputs are requested now too (the prescribed values, needed only on
+    // flagged DOFs, are read at the update)
+    constexpr bool PRE = LPN >= 4;
+    int pfl = 0;
+    double pu = 0.0, pud = 0.0, pudd = 0.0, pdens = 1.0;
+    if (PRE && valid && sub < 3) {
+        const size_t k = 3 * (size_t)i + sub;
+        pfl = a.flags[i];
+        pu = a.u[k];
+        if (a.integrator != PD_EULER) {
+            pud = a.ud[k];
+            pdens = a.dens[k];
+            if (a.integrator == PD_VELOCITY_VERLET)
+                pudd = a.udd[k];
+        }
     }
 
     // stage the halo: piece x copies 16-byte piece x%4 of record x/4, SP
@@ -578,7 +590,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
 
     // the node's lanes share its three components (after the xor reduction
     // every lane holds the sums), so the per-node loads and stores coalesce
-    const int fl = a.flags[i];
+    const int fl = PRE ? pfl : a.flags[i];
     const size_t b3 = 3 * (size_t)i;
     for (int d = sub; d < 3; d += LPN) {
         const size_t k = b3 + d;
@@ -589,14 +601,15 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         a.body_force[k] = bf;
         a.force[k] = F;
         const bool fixed = (fl >> d) & 1;
-        double u = a.u[k];
+        double u = PRE ? pu : a.u[k];
         if (a.integrator == PD_EULER) {
             // peridynamics.pyx:179-185, cl/euler.cl:21-23
             u = fixed ? dmag * a.bcv[k] : u + a.dt * F;
         } else if (a.integrator == PD_EULER_CROMER) {
             // cl/euler_cromer.cl:29-34
-            double ud = a.ud[k];
-            const double udd = (F - a.damping * ud) / a.dens[k];
+            double ud = PRE ? pud : a.ud[k];
+            const double dens = PRE ? pdens : a.dens[k];
+            const double udd = (F - a.damping * ud) / dens;
             ud = ud + udd * a.dt;
             a.udd[k] = udd;
             a.ud[k] = ud;
@@ -604,8 +617,10 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         } else {
             // cl/velocity_verlet.cl:29-35
             const double hdt = a.dt / 2;
-            const double ud1 = a.ud[k] + hdt * a.udd[k];
-            const double udd1 = (F - a.damping * ud1) / a.dens[k];
+            const double ud1 =
+                (PRE ? pud : a.ud[k]) + hdt * (PRE ? pudd : a.udd[k]);
+            const double dens = PRE ? pdens : a.dens[k];
+            const double udd1 = (F - a.damping * ud1) / dens;
             const double ud = ud1 + hdt * udd1;
             a.ud[k] = ud;
             a.udd[k] = udd1;
