@@ -424,3 +424,43 @@ def test_restart_is_bitwise_and_stateless(golden):
         r2 = _state(two)
         for key in ("u", "force", "damage", "nlist", "n_neigh"):
             np.testing.assert_array_equal(r1[key], r2[key])
+
+
+# --------------------------------------------------------------------------
+# full-size workloads: size-independent properties
+# --------------------------------------------------------------------------
+
+def _momentum_defect(st, volume):
+    """|sum_i V_i F_i| / sum_i |V_i F_i| over the bond forces (body_force):
+    each bond adds f V_j to i and -f V_i to j, so the sum vanishes."""
+    vf = volume[:, None] * st["body_force"]
+    return np.abs(vf.sum(axis=0)).max() / np.abs(vf).sum(axis=0).max()
+
+
+@pytest.mark.parametrize("name", ["beam_vv", "composite_ec"])
+def test_full_size_momentum_and_restart(name):
+    w = getattr(configs, name)()
+    integ = configs.make_integrator(w)
+    d, f = w.magnitudes(12)
+    integ.run(6, d[:6], f[:6])
+    mid = _state(integ)
+    defect = _momentum_defect(mid, w.volume)
+    print(f"{name}: momentum defect {defect:.3e}")
+    assert defect < 1e-13
+    integ.run(6, d[6:], f[6:])
+    end = _state(integ)
+    # restarting from the mid-run state reproduces the second half bitwise
+    again = configs.make_integrator(w)
+    again.create_buffers(mid["nlist"], mid["n_neigh"], w.bond_stiffness,
+                         w.critical_stretch, w.plus_cs, mid["u"], mid["ud"],
+                         mid["udd"], None, None, None,
+                         integ.regimes() if w.nregimes > 1 else None,
+                         w.nregimes, w.nbond_types)
+    again.run(6, d[6:], f[6:])
+    st = _state(again)
+    np.testing.assert_array_equal(st["u"], end["u"])
+    np.testing.assert_array_equal(st["nlist"], end["nlist"])
+    # another launch shape sums each node's bonds in another order only
+    other = configs.make_integrator(w, lanes_per_node=2, tile_nodes=256)
+    other.run(12, d, f)
+    assert pc.rel_l2(_state(other)["u"], end["u"]) < 1e-12
