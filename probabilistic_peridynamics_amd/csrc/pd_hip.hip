@@ -1700,6 +1700,10 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
             return fail(PD_E_LIMIT, "internal: tile halo over capacity");
 
     HIPCHK(hipStreamSynchronize(s->stream));
+    // from here the device state is replaced: until it is whole again no
+    // step may run (a failure below leaves the simulation without buffers)
+    s->has_buffers = false;
+    adjoint_release(s);
     auto chk = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && rc == PD_OK)
             rc = fail(PD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
