@@ -113,6 +113,52 @@ def copy_probe(mib):
     return 0
 
 
+def cpu_multicore(w, budget_s):
+    """The oracle's per-node (gather) formulation of the same step -- the
+    slot-layout bond force of cl/peridynamics.cl restated with the Cython
+    stretch, then the integrator's update -- on all host cores given to this
+    process (at most 16), whole steps until budget_s has passed."""
+    import oracle
+    L = oracle.lib()
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    L.oracle_set_threads(cores)
+    n, k = w.nnodes, w.max_neighbours
+    nlist = np.array(w.nlist, dtype=np.int32)
+    u, ud, udd = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    r = np.zeros((n, 3))
+    force, body = np.zeros((n, 3)), np.zeros((n, 3))
+    d, f = w.magnitudes(10000)
+    live = int(w.n_neigh.sum())
+    D, I = oracle._d, oracle._i
+    upd = (L.oracle_update_velocity_verlet
+           if w.integrator == "velocity_verlet"
+           else L.oracle_update_euler_cromer)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        L.oracle_positions(n, D(w.coords), D(u), D(r))
+        L.oracle_slot_bond_force(
+            n, k, D(r), D(w.coords), I(nlist), D(w.volume),
+            float(w.bond_stiffness), float(w.critical_stretch), 0, None,
+            None, None, None, None, None, None, 1, I(w.force_bc_types),
+            D(w.force_bc_values), f[steps], D(force), D(body))
+        upd(n, D(force), D(u), D(ud), D(udd), I(w.bc_types),
+            D(w.bc_values), D(w.densities), d[steps], w.damping, w.dt)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    elapsed = time.perf_counter() - t0
+    L.oracle_set_threads(1)
+    return {
+        "value": live * steps / elapsed, "unit": "bond-updates/s",
+        "cores": cores, "kind": "port",
+        "sample": (f"{steps} full steps of the {n}-node workload "
+                   f"({elapsed:.1f} s): the per-node slot-layout bond force "
+                   "(cl/peridynamics.cl semantics, Cython stretch) and the "
+                   f"{w.integrator} update restated in C (gcc -O3, OpenMP)"),
+    }
+
+
 def uq_bench(nsamples, steps, tune_samples):
     """Batched vs sequential sample-steps/s on the UQ batch."""
     from probabilistic_peridynamics_amd import autotune, batch, configs
@@ -247,6 +293,7 @@ def cpu_baseline(w, budget_s):
     (peridynamics.pyx order, one thread) and the integrator's update, whole
     steps until budget_s has passed (at least one)."""
     import oracle
+    oracle.lib().oracle_set_threads(1)
     n = w.nnodes
     coords = w.coords
     nlist = np.array(w.nlist, dtype=np.int32)
@@ -375,6 +422,8 @@ def main():
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(w, args.cpu_seconds)
         cpu["threads_available"] = os.cpu_count()
+        if w.integrator != "euler":
+            cpu["multicore"] = cpu_multicore(w, args.cpu_seconds)
 
     name, arch = _lib.device_name(0)
     out = {

@@ -73,6 +73,8 @@ def lib():
                 ctypes.c_int, _D, _D, _D, _D, _I, _D, _D, ctypes.c_double,
                 ctypes.c_double, ctypes.c_double]),
             "oracle_positions": (None, [ctypes.c_int, _D, _D, _D]),
+            "oracle_set_threads": (None, [ctypes.c_int]),
+            "oracle_max_threads": (ctypes.c_int, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -33,6 +33,30 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the per-node loops below (slot bond force, updates,
+ * positions).  Every node's arithmetic is the same at any thread count;
+ * the Cython break_bonds / bond_force scatter stays single-threaded. */
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int oracle_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ---------------------------------------------------------------------- */
 /* (1) spatial.pyx                                                         */
@@ -172,6 +196,7 @@ void oracle_update_displacement(int nnodes, double *u,
                                 const double *force, double bc_scale,
                                 double dt)
 {
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t i = 0; i < 3 * (size_t)nnodes; ++i) {
         if (bc_types[i] == 0)
             u[i] = u[i] + dt * force[i];
@@ -235,6 +260,7 @@ void oracle_slot_bond_force(
     int nregimes, const int *force_bc_types, const double *force_bc_values,
     double force_bc_scale, double *force, double *body_force)
 {
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (int i = 0; i < nnodes; ++i) {
         double acc[3] = {0.0, 0.0, 0.0};
         for (int p = 0; p < max_neigh; ++p) {
@@ -316,6 +342,7 @@ void oracle_update_euler_cromer(int nnodes, const double *force, double *u,
                                 const double *densities, double bc_scale,
                                 double damping, double dt)
 {
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q) {
         double uddi = (force[q] - damping * ud[q]) / densities[q];
         udd[q] = uddi;
@@ -333,6 +360,7 @@ void oracle_update_velocity_verlet(int nnodes, const double *force, double *u,
                                    const double *densities, double bc_scale,
                                    double damping, double dt)
 {
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q) {
         double ud1 = ud[q] + (dt / 2) * udd[q];
         double udd1 = (force[q] - damping * ud1) / densities[q];
@@ -348,6 +376,7 @@ void oracle_update_velocity_verlet(int nnodes, const double *force, double *u,
 void oracle_positions(int nnodes, const double *r0, const double *u,
                       double *r)
 {
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q)
         r[q] = r0[q] + u[q];
 }
