@@ -217,6 +217,14 @@ int pd_neighbour_list(int nnodes, const double *coords, double horizon,
                       int *family, int *nlist, int max_neighbours,
                       int pad_value);
 
+/* pd_neighbour_list computed on a GPU: the same family, rows and padding
+ * (the host bins the nodes into cells; the GPU searches, fills and sorts the
+ * rows).  max_neighbours at most 4096.  Replaces, like pd_neighbour_list,
+ * the KD-tree query of model.py:536-597. */
+int pd_neighbour_list_device(int device, int nnodes, const double *coords,
+                             double horizon, int *family, int *nlist,
+                             int max_neighbours, int pad_value);
+
 #ifdef __cplusplus
 }
 #endif

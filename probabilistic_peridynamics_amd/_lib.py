@@ -61,6 +61,9 @@ SIGNATURES = {
     "pd_synchronize": (ctypes.c_int, [_SIM]),
     "pd_tile_stats": (ctypes.c_int, [_SIM, _I, _I, _I, _LL, _LL]),
     "pd_destroy": (None, [_SIM]),
+    "pd_neighbour_list_device": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, _D, ctypes.c_double, _I, _I, ctypes.c_int,
+        ctypes.c_int]),
     "pd_bond_check": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_double,
         ctypes.c_int, _D, _D, _D, _D, _I]),
@@ -164,17 +167,29 @@ def device_name(device=0):
     return name.value.decode(), arch.value.decode()
 
 
-def family_count(coords, horizon):
-    """Number of nodes within ``horizon`` of each node (family)."""
+def _nl_call(n, coords, horizon, family, nlist, k, pad, device):
+    if device is None:
+        return lib().pd_neighbour_list(n, dptr(coords), float(horizon),
+                                       iptr(family), iptr(nlist), k, pad)
+    return lib().pd_neighbour_list_device(int(device), n, dptr(coords),
+                                          float(horizon), iptr(family),
+                                          iptr(nlist), k, pad)
+
+
+def family_count(coords, horizon, device=None):
+    """Number of nodes within ``horizon`` of each node (family); on the host
+    (``device=None``) or on that GPU."""
     coords = as_f64(coords)
     family = np.zeros(coords.shape[0], dtype=np.int32)
-    check(lib().pd_neighbour_list(coords.shape[0], dptr(coords),
-                                  float(horizon), iptr(family), None, 0, 0))
+    check(_nl_call(coords.shape[0], coords, horizon, family, None, 0, 0,
+                   device))
     return family
 
 
-def neighbour_list(coords, horizon, max_neighbours=None, pad_value=0):
-    """(family, nlist) of a point cloud; rows ascending, padded.
+def neighbour_list(coords, horizon, max_neighbours=None, pad_value=0,
+                   device=None):
+    """(family, nlist) of a point cloud; rows ascending, padded.  On the host
+    (``device=None``) or on that GPU, with the same result.
 
     ``max_neighbours`` defaults to ``family.max()``.
     """
@@ -182,12 +197,11 @@ def neighbour_list(coords, horizon, max_neighbours=None, pad_value=0):
     n = coords.shape[0]
     family = np.zeros(n, dtype=np.int32)
     if max_neighbours is None:
-        family = family_count(coords, horizon)
+        family = family_count(coords, horizon, device)
         max_neighbours = int(family.max())
     nlist = np.empty((n, int(max_neighbours)), dtype=np.int32)
-    check(lib().pd_neighbour_list(n, dptr(coords), float(horizon),
-                                  iptr(family), iptr(nlist),
-                                  int(max_neighbours), int(pad_value)))
+    check(_nl_call(n, coords, horizon, family, nlist, int(max_neighbours),
+                   int(pad_value), device))
     return family, nlist
 
 

@@ -1,7 +1,7 @@
 """Build libpd_hip.so in-tree with hipcc for gfx950.
 
 ``python -m probabilistic_peridynamics_amd.build`` compiles
-``csrc/pd_hip.hip`` into ``probabilistic_peridynamics_amd/libpd_hip.so``.  It
+``csrc/pd_hip.hip`` and ``csrc/pd_nlist.hip`` into ``probabilistic_peridynamics_amd/libpd_hip.so``.  It
 needs hipcc but no GPU.
 
 Flags:
@@ -18,7 +18,7 @@ import subprocess
 import sys
 
 HERE = pathlib.Path(__file__).resolve().parent
-SOURCES = [HERE / "csrc" / "pd_hip.hip"]
+SOURCES = [HERE / "csrc" / "pd_hip.hip", HERE / "csrc" / "pd_nlist.hip"]
 TARGET = HERE / "libpd_hip.so"
 ARCH = os.environ.get("PD_OFFLOAD_ARCH", "gfx950")
 

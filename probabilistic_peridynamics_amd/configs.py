@@ -105,18 +105,28 @@ class Workload:
                 np.full(steps, self.force_scale, dtype=np.float64))
 
 
+def _search_device():
+    """GPU 0 for the neighbour search when there is one, else the host."""
+    try:
+        return 0 if _lib.device_count() > 0 else None
+    except (OSError, _lib.PDError):
+        return None
+
+
 def neighbours(coords, horizon, layout="slot"):
     """(family, nlist, n_neigh) in the reference's layouts: "slot" pads a
     power-of-two width with -1 (model.py:575-579), "compact" pads width
-    family.max() with 0 (model.py:580-582)."""
-    family = _lib.family_count(coords, horizon)
+    family.max() with 0 (model.py:580-582).  The search runs on the GPU when
+    there is one (the same result as the host search)."""
+    device = _search_device()
+    family = _lib.family_count(coords, horizon, device)
     if layout == "slot":
         k = 1 << (int(family.max()) - 1).bit_length()
         pad = -1
     else:
         k = int(family.max())
         pad = 0
-    family, nlist = _lib.neighbour_list(coords, horizon, k, pad)
+    family, nlist = _lib.neighbour_list(coords, horizon, k, pad, device)
     return family, nlist, family.copy()
 
 

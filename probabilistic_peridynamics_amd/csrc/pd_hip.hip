@@ -104,6 +104,9 @@ int fail(int code, const std::string &msg)
 
 }  // namespace
 
+// Error reporting for the library's other translation units.
+int pd_fail(int code, const char *msg) { return fail(code, msg); }
+
 namespace pdk {
 
 constexpr uint32_t DEAD = 0x80000000u;

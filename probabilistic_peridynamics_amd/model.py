@@ -276,8 +276,11 @@ class Model(object):
 
         Layout as model.py:575-582: a truthy context gives a power-of-two
         width padded with -1, otherwise width family.max() padded with 0.
+        With a HIP context the search runs on its GPU (same result as the
+        host search).
         """
-        family = _lib.family_count(coords, horizon)
+        device = getattr(context, "device", None) if context else None
+        family = _lib.family_count(coords, horizon, device)
         if context:
             max_neighbours = np.intc(1 << (int(family.max() - 1)).bit_length())
             pad = -1
@@ -285,7 +288,7 @@ class Model(object):
             max_neighbours = np.intc(family.max())
             pad = 0
         family, nlist = _lib.neighbour_list(coords, horizon,
-                                            int(max_neighbours), pad)
+                                            int(max_neighbours), pad, device)
         n_neigh = family.copy()
         if initial_crack is not None:
             if callable(initial_crack):
