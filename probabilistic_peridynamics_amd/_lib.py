@@ -167,6 +167,14 @@ def device_name(device=0):
     return name.value.decode(), arch.value.decode()
 
 
+def device_available(device=0):
+    """Whether HIP device ``device`` exists (False without the runtime)."""
+    try:
+        return 0 <= int(device) < device_count()
+    except (OSError, PDError):
+        return False
+
+
 def _nl_call(n, coords, horizon, family, nlist, k, pad, device):
     if device is None:
         return lib().pd_neighbour_list(n, dptr(coords), float(horizon),

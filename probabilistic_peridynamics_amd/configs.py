@@ -107,10 +107,7 @@ class Workload:
 
 def _search_device():
     """GPU 0 for the neighbour search when there is one, else the host."""
-    try:
-        return 0 if _lib.device_count() > 0 else None
-    except (OSError, _lib.PDError):
-        return None
+    return 0 if _lib.device_available(0) else None
 
 
 def neighbours(coords, horizon, layout="slot"):

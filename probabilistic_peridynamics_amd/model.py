@@ -280,6 +280,8 @@ class Model(object):
         host search).
         """
         device = getattr(context, "device", None) if context else None
+        if device is not None and not _lib.device_available(device):
+            device = None
         family = _lib.family_count(coords, horizon, device)
         if context:
             max_neighbours = np.intc(1 << (int(family.max() - 1)).bit_length())
