@@ -29,9 +29,11 @@
 // a tile's nodes are a contiguous range of p, and a halo is a few contiguous
 // runs, so per-node loads and stores coalesce.  Inputs are permuted into
 // this order on upload and outputs back on download.
-//   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, V | rx, ry, rz, c) with
-//              r = r0 + u and c the per-node stiffness (or 0); ping-pong,
-//              with r0, V and c static in both.
+//   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, w | rx, ry, rz, v) with
+//              r = r0 + u and the bond loop's weights w (4kV for the scalar
+//              model, 2V for a stiffness field c, V for n-linear models) and
+//              v (2cV for a field, else 0); ping-pong, with r0, w and v
+//              static in both.
 //   tiles[T]   TileInfo: node range [node_off, node_off + nodes) of p, halo
 //              range in halo[], first chunk of the tile's bond slots, chunk
 //              rows.
@@ -148,7 +150,7 @@ struct StepArgs {
     int integrator, layout, has_corr;
     unsigned absmask;  // 0x7fffffff: break on |s|; 0xffffffff: on s
     int nregimes, ntab;
-    double dt, damping, s0h, s0h_guard, k4;
+    double dt, damping, s0h, s0h_guard;
     const NodeRec *rec_old;
     NodeRec *rec_new;
     const TileInfo *tiles;
@@ -165,6 +167,7 @@ struct StepArgs {
     const long long *chunk0;
     const int *stride;
     int *nlist_ref, *n_neigh;
+    const double *cnode;  // node stiffness c (DM_FIELD)
     int *brk;  // if set: the step at which each slot's bond broke
     int step;  // index of this step (for brk)
 };
@@ -293,10 +296,12 @@ constexpr unsigned piece_bytes()
 }
 
 // W bonds of node i at slots s0 .. s0+W-1: ev their halo indices (LOC_DEAD:
-// broken or empty), st their (bond type << 8 | regime) for DM_NLIN.
+// broken or empty), st their (bond type << 8 | regime) for DM_NLIN, ci node
+// i's stiffness for DM_FIELD.
 template <int DM, int W>
 __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
                                       const double4 &r0i, const double4 &ri,
+                                      double ci,
                                       const unsigned *ev,
                                       const unsigned short *st, size_t s0,
                                       const double *ts, const double *tk,
@@ -336,8 +341,9 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
         sqrt_core(qq, l, hl[q]);
         sqrt_core(q0, l0, h0);
         sh[q] = (l - l0) * h0;  // s/2, from h0 ~ 0.5/l0
-        // k = (c_i + c_j)/2 for a field: s k/l V_j = (s/2)(0.5/l)(2(c_i+c_j)V_j)
-        w[q] = DM == DM_FIELD ? (ri.w + p3.y) * 2.0 * p1.y : p1.y;
+        // k = (c_i + c_j)/2 for a field: s k/l V_j = (s/2)(0.5/l) w with
+        // w = 2(c_i+c_j)V_j = c_i (2 V_j) + (2 c_j V_j), the staged pair
+        w[q] = DM == DM_FIELD ? __builtin_fma(ci, p1.y, p3.y) : p1.y;
         const bool fast = core_range(qq) & core_range(q0);
         bool near;
         if (DM != DM_NLIN) {
@@ -433,8 +439,9 @@ struct Chunk {
 };
 
 // A chunk of four bonds of node i, slots s0 .. s0+3.  The pieces of halo
-// entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, c), with w = 4*k*V for
-// DM_PMB and w = V otherwise.  The bonds go two at a time: their reads are
+// entry h are (r0x, r0y) (r0z, w) (rx, ry) (rz, v), as the records: w =
+// 4 k V for DM_PMB, 2 V for DM_FIELD and V for DM_NLIN; v = 2 c V for
+// DM_FIELD.  The bonds go two at a time: their reads are
 // issued first and the arithmetic is straight-line code (a dead slot
 // computes on the zero record and is masked), so the two chains interleave.
 // (Four at a time is no faster and needs 40 more registers, which the
@@ -444,6 +451,7 @@ struct Chunk {
 template <int DM>
 __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
                                        const double4 &r0i, const double4 &ri,
+                                       double ci,
                                        const Chunk &ch, size_t s0,
                                        const double *ts, const double *tk,
                                        const double *tp, double &fx,
@@ -451,10 +459,10 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
 {
     const unsigned ev[4] = {ch.loc.x, ch.loc.y, ch.loc.z, ch.loc.w};
     const unsigned short st[4] = {ch.st.x, ch.st.y, ch.st.z, ch.st.w};
-    bonds<DM, 2>(a, ldsb, r0i, ri, ev, st, s0, ts, tk, tp, fx, fy, fz,
+    bonds<DM, 2>(a, ldsb, r0i, ri, ci, ev, st, s0, ts, tk, tp, fx, fy, fz,
                  nbroken);
-    bonds<DM, 2>(a, ldsb, r0i, ri, ev + 2, st + 2, s0 + 2, ts, tk, tp, fx, fy,
-                 fz, nbroken);
+    bonds<DM, 2>(a, ldsb, r0i, ri, ci, ev + 2, st + 2, s0 + 2, ts, tk, tp, fx,
+                 fy, fz, nbroken);
 }
 
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
@@ -503,6 +511,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         ri = pi->b;
         cnt = a.cnt[i];
     }
+    const double ci = DM == DM_FIELD && valid ? a.cnode[i] : 0.0;
     // with 4 or more lanes per node, lane d < 3 updates component d: its
     // inputs are requested now too (the prescribed values, needed only on
     // flagged DOFs, are read at the update)
@@ -521,9 +530,9 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         }
     }
 
-    // stage the halo: piece x copies 16-byte piece x%4 of record x/4, SP
-    // pieces per thread in flight at once (loads clamped in bounds, stores
-    // predicated); the r0/V piece of DM_PMB becomes (r0z, 4 k V)
+    // stage the halo: piece x copies 16-byte piece x%4 of record x/4 into
+    // entry x/4 + 1, SP pieces per thread in flight at once (loads clamped in
+    // bounds, stores predicated)
     constexpr int SP = 10;
     constexpr unsigned PB = piece_bytes<DM>();
     const int np = 4 * ti.halo;
@@ -539,9 +548,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         for (int k = 0; k < SP; ++k) {
             const int x = x0 + k * (int)blockDim.x;
             if (x < np) {
-                double2 w = v[k];
-                if (DM == DM_PMB && (x & 3) == 1)
-                    w.y = a.k4 * w.y;
+                const double2 w = v[k];
                 *reinterpret_cast<double2 *>(
                     ldsb + (x & 3) * PB + 16u * (unsigned)((x >> 2) + 1)) = w;
             }
@@ -568,7 +575,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         for (int c = sub; c < nch; c += LPN) {
             const Chunk nx = chunk_at(c + DEPTH * LPN);
             const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
-            chunk4<DM>(a, ldsb, r0i, ri, q0, s0, ts, tk, tp, fx, fy, fz,
+            chunk4<DM>(a, ldsb, r0i, ri, ci, q0, s0, ts, tk, tp, fx, fy, fz,
                        nbroken);
             q0 = q1;
             q1 = q2;
@@ -681,20 +688,22 @@ __global__ void k_bond_check(int n, const double *ri, const double *rj,
     broken[t] = !(as < s0h);
 }
 
-// Records of both ping-pong buffers from r0, V, u and c: a = (r0, V),
-// b = (r0 + u, c).
-__global__ void k_records(int n, const double *r0, const double *vol,
-                          const double *u, const double *c, NodeRec *rec0,
-                          NodeRec *rec1)
+// Records of both ping-pong buffers from r0, V, u and c: a = (r0, wscale V),
+// b = (r0 + u, v) with v = 2 c V for a field c, else 0.  The weights are
+// the factors the bond loop multiplies by: wscale = 4k for the scalar
+// model, 2 for the field model, 1 for the n-linear model.
+__global__ void k_records(int n, double wscale, const double *r0,
+                          const double *vol, const double *u, const double *c,
+                          NodeRec *rec0, NodeRec *rec1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const size_t b = 3 * (size_t)i;
     NodeRec r;
-    r.a = make_double4(r0[b], r0[b + 1], r0[b + 2], vol[i]);
+    r.a = make_double4(r0[b], r0[b + 1], r0[b + 2], vol[i] * wscale);
     r.b = make_double4(r0[b] + u[b], r0[b + 1] + u[b + 1], r0[b + 2] + u[b + 2],
-                       c ? c[i] : 0.0);
+                       c ? 2.0 * c[i] * vol[i] : 0.0);
     rec0[i] = r;
     rec1[i] = r;
 }
@@ -1094,7 +1103,6 @@ StepArgs make_args(pd_sim *s)
     a.damping = s->damping;
     a.s0h = s->s0 * 0.5;
     a.s0h_guard = GUARD * a.s0h;
-    a.k4 = s->k * 4.0;
     a.rec_old = s->rec[s->cur].p;
     a.rec_new = s->rec[1 - s->cur].p;
     a.tiles = s->tiles.p;
@@ -1121,6 +1129,7 @@ StepArgs make_args(pd_sim *s)
     a.stride = s->stride.p;
     a.nlist_ref = s->nlist_ref.p;
     a.n_neigh = s->n_neigh.p;
+    a.cnode = s->cnode.p;
     a.brk = s->adj_recording ? s->brk.p : nullptr;
     a.step = s->step_counter;
     return a;
@@ -1230,8 +1239,11 @@ int launch_step(pd_sim *s, double dmag, double fmag)
 int refresh_records(pd_sim *s)
 {
     const int blocks = (s->n + 255) / 256;
+    const int dm = dm_of(s);
+    const double wscale =
+        dm == DM_PMB ? 4.0 * s->k : (dm == DM_FIELD ? 2.0 : 1.0);
     hipLaunchKernelGGL(k_records, dim3(blocks), dim3(256), 0, s->stream, s->n,
-                       s->r0.p, s->vol.p, s->u.p,
+                       wscale, s->r0.p, s->vol.p, s->u.p,
                        s->has_cnode ? s->cnode.p : nullptr, s->rec[0].p,
                        s->rec[1].p);
     HIPCHK(hipGetLastError());
