@@ -225,6 +225,52 @@ int pd_neighbour_list_device(int device, int nnodes, const double *coords,
                              double horizon, int *family, int *nlist,
                              int max_neighbours, int pad_value);
 
+/* Stiffness-correction factors, the four functions of peripy/correction.pyx
+ * with their argument meaning and order, after a leading `device` (a GPU
+ * index, or -1 to run the same code on the host).  Each multiplies the
+ * (nnodes, max_neighbours) float64 array it is given, in place, for every
+ * live bond (slot s < n_neigh[i]) exactly as the reference's loop does: a
+ * pair i < j multiplies slot (i, s) and the first slot of row j that holds
+ * i; factors are bitwise the reference's.  Slots naming a node outside
+ * [0, nnodes) or i itself are left alone.  n_neigh must lie in
+ * [0, max_neighbours].  r0 is (nnodes, 3). */
+
+/* correction.pyx:208-277 set_micromodulus_function: conical
+ * (h - l0)/h for l0 <= h, else 0; factor 1 when micromodulus_function != 0 */
+int pd_set_micromodulus_function(int device, int nnodes, int max_neighbours,
+                                 double *micromodulus_values,
+                                 const double *r0, const int *nlist,
+                                 const int *n_neigh, double horizon,
+                                 int micromodulus_function);
+
+/* correction.pyx:123-205 set_volume_correction (cPA_HHB): 1 for
+ * l0 <= h - r, (h + r - l0)/(2r) for l0 <= h + r, else 0; factor 1 when
+ * volume_correction != 0 */
+int pd_set_volume_correction(int device, int nnodes, int max_neighbours,
+                             double *volume_corrections, const double *r0,
+                             const int *nlist, const int *n_neigh,
+                             double horizon, double node_radius,
+                             int volume_correction);
+
+/* correction.pyx:5-56 set_imprecise_surface_correction:
+ * 2*V_bulk / (n_neigh[i]*V_avg + n_neigh[j]*V_avg) */
+int pd_set_imprecise_surface_correction(int device, int nnodes,
+                                        int max_neighbours,
+                                        double *stiffness_corrections,
+                                        const int *nlist, const int *n_neigh,
+                                        double average_volume,
+                                        double family_volume_bulk);
+
+/* correction.pyx:59-120 set_precise_surface_correction:
+ * 2*V_bulk / (V_fam,i + V_fam,j), V_fam,i the in-order sum of the volumes
+ * of i's live neighbours */
+int pd_set_precise_surface_correction(int device, int nnodes,
+                                      int max_neighbours,
+                                      double *stiffness_corrections,
+                                      const int *nlist, const int *n_neigh,
+                                      const double *volume,
+                                      double family_volume_bulk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,18 @@ def lib():
                 ctypes.c_int, _D, _D, _D, _D, _I, _D, _D, ctypes.c_double,
                 ctypes.c_double, ctypes.c_double]),
             "oracle_positions": (None, [ctypes.c_int, _D, _D, _D]),
+            "oracle_set_micromodulus_function": (None, [
+                ctypes.c_int, ctypes.c_int, _D, _D, _I, _I, ctypes.c_double,
+                ctypes.c_int]),
+            "oracle_set_volume_correction": (None, [
+                ctypes.c_int, ctypes.c_int, _D, _D, _I, _I, ctypes.c_double,
+                ctypes.c_double, ctypes.c_int]),
+            "oracle_set_imprecise_surface_correction": (None, [
+                ctypes.c_int, ctypes.c_int, _D, _I, _I, ctypes.c_double,
+                ctypes.c_double]),
+            "oracle_set_precise_surface_correction": (None, [
+                ctypes.c_int, ctypes.c_int, _D, _I, _I, _D, ctypes.c_double,
+                _D]),
             "oracle_set_threads": (None, [ctypes.c_int]),
             "oracle_max_threads": (ctypes.c_int, []),
         }
@@ -166,6 +178,50 @@ def create_crack(crack, nlist, n_neigh):
                                  .reshape(-1, 2))
     lib().oracle_create_crack(crack.shape[0], _i(crack), nlist.shape[1],
                               _i(nlist), _i(n_neigh))
+
+
+# --------------------------------------------------------------------------
+# Function-level mirrors of peripy.correction (all mutate their first array)
+# --------------------------------------------------------------------------
+
+def set_micromodulus_function(micromodulus_values, r0, nlist, n_neigh,
+                              horizon, micromodulus_function):
+    """correction.pyx:208-277."""
+    n, k = nlist.shape
+    r0 = np.ascontiguousarray(r0, dtype=np.float64)
+    lib().oracle_set_micromodulus_function(
+        n, k, _d(micromodulus_values), _d(r0), _i(nlist), _i(n_neigh),
+        float(horizon), int(micromodulus_function))
+
+
+def set_volume_correction(volume_corrections, r0, nlist, n_neigh, horizon,
+                          node_radius, volume_correction):
+    """correction.pyx:123-205."""
+    n, k = nlist.shape
+    r0 = np.ascontiguousarray(r0, dtype=np.float64)
+    lib().oracle_set_volume_correction(
+        n, k, _d(volume_corrections), _d(r0), _i(nlist), _i(n_neigh),
+        float(horizon), float(node_radius), int(volume_correction))
+
+
+def set_imprecise_surface_correction(stiffness_corrections, nlist, n_neigh,
+                                     average_volume, family_volume_bulk):
+    """correction.pyx:5-56."""
+    n, k = nlist.shape
+    lib().oracle_set_imprecise_surface_correction(
+        n, k, _d(stiffness_corrections), _i(nlist), _i(n_neigh),
+        float(average_volume), float(family_volume_bulk))
+
+
+def set_precise_surface_correction(stiffness_corrections, nlist, n_neigh,
+                                   volume, family_volume_bulk):
+    """correction.pyx:59-120."""
+    n, k = nlist.shape
+    volume = np.ascontiguousarray(volume, dtype=np.float64)
+    scratch = np.empty(n, dtype=np.float64)
+    lib().oracle_set_precise_surface_correction(
+        n, k, _d(stiffness_corrections), _i(nlist), _i(n_neigh), _d(volume),
+        float(family_volume_bulk), _d(scratch))
 
 
 # --------------------------------------------------------------------------

@@ -380,3 +380,152 @@ void oracle_positions(int nnodes, const double *r0, const double *u,
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q)
         r[q] = r0[q] + u[q];
 }
+
+/* ---------------------------------------------------------------------- */
+/* (1) correction.pyx: stiffness-correction factors (set-up)               */
+/* ---------------------------------------------------------------------- */
+
+/* The loop every function of correction.pyx shares (:32-56, :97-120,
+ * :155-183, :236-262): rows i ascending, live slots in order; for a
+ * neighbour j > i the pair's factor f multiplies slot (i, neigh) and the
+ * FIRST slot of row j (within n_neigh[j]) that holds i.  Slots whose
+ * neighbour is out of range are skipped (the Cython reads out of bounds). */
+typedef double (*oracle_pair_fn)(int i, int j, const void *ctx);
+
+static void oracle_pairwise(int nnodes, int max_neigh, double *corr,
+                            const int *nlist, const int *n_neigh,
+                            oracle_pair_fn fn, const void *ctx)
+{
+    for (int i = 0; i < nnodes; ++i) {
+        for (int neigh = 0; neigh < n_neigh[i]; ++neigh) {
+            int j = nlist[(size_t)i * max_neigh + neigh];
+            if (j < 0 || j >= nnodes || !(i < j))
+                continue;
+            double f = fn(i, j, ctx);
+            corr[(size_t)i * max_neigh + neigh] *= f;
+            for (int jn = 0; jn < n_neigh[j]; ++jn) {
+                if (nlist[(size_t)j * max_neigh + jn] == i) {
+                    corr[(size_t)j * max_neigh + jn] *= f;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+struct oracle_geom {
+    const double *r0;
+    double horizon, node_radius;
+    int flag;
+};
+
+/* correction.pyx:265-277 (cmicromodulus_connical):
+ * l0 <= h ? (h - l0)/h : 0; any flag but 0 gives 1.00 (:246-251). */
+static double oracle_conical(int i, int j, const void *ctx)
+{
+    const struct oracle_geom *g = (const struct oracle_geom *)ctx;
+    if (g->flag != 0)
+        return 1.00;
+    double l0 = oracle_euclid(&g->r0[3 * (size_t)i], &g->r0[3 * (size_t)j]);
+    if (l0 <= g->horizon)
+        return (g->horizon - l0) / g->horizon;
+    return 0.00;
+}
+
+/* correction.pyx:186-205 (cPA_HHB): l0 <= h - r -> 1; l0 <= h + r ->
+ * (h + r - l0)/(2r); else 0.  Any flag but 0 gives 1.00 (:166-172). */
+static double oracle_pa_hhb(int i, int j, const void *ctx)
+{
+    const struct oracle_geom *g = (const struct oracle_geom *)ctx;
+    if (g->flag != 0)
+        return 1.00;
+    double l0 = oracle_euclid(&g->r0[3 * (size_t)i], &g->r0[3 * (size_t)j]);
+    double h = g->horizon, r = g->node_radius;
+    if (l0 <= h - r)
+        return 1.00;
+    if (l0 <= h + r)
+        return (h + r - l0) / (2.0 * r);
+    return 0.00;
+}
+
+struct oracle_surface {
+    const double *family_volume; /* NULL: n_neigh[i] * average */
+    const int *n_neigh;
+    double average, bulk;
+};
+
+/* correction.pyx:42-45, :107-109: 2*V_bulk / (V_fam,i + V_fam,j) */
+static double oracle_volume_method(int i, int j, const void *ctx)
+{
+    const struct oracle_surface *s = (const struct oracle_surface *)ctx;
+    double vi, vj;
+    if (s->family_volume) {
+        vi = s->family_volume[i];
+        vj = s->family_volume[j];
+    } else {
+        vi = s->n_neigh[i] * s->average;
+        vj = s->n_neigh[j] * s->average;
+    }
+    return 2. * s->bulk / (vi + vj);
+}
+
+/* correction.pyx:208-262 (set_micromodulus_function) */
+void oracle_set_micromodulus_function(int nnodes, int max_neigh,
+                                      double *values, const double *r0,
+                                      const int *nlist, const int *n_neigh,
+                                      double horizon, int flag)
+{
+    struct oracle_geom g = {r0, horizon, 0.0, flag};
+    oracle_pairwise(nnodes, max_neigh, values, nlist, n_neigh,
+                    oracle_conical, &g);
+}
+
+/* correction.pyx:123-183 (set_volume_correction) */
+void oracle_set_volume_correction(int nnodes, int max_neigh, double *values,
+                                  const double *r0, const int *nlist,
+                                  const int *n_neigh, double horizon,
+                                  double node_radius, int flag)
+{
+    struct oracle_geom g = {r0, horizon, node_radius, flag};
+    oracle_pairwise(nnodes, max_neigh, values, nlist, n_neigh,
+                    oracle_pa_hhb, &g);
+}
+
+/* correction.pyx:5-56 (set_imprecise_surface_correction) */
+void oracle_set_imprecise_surface_correction(int nnodes, int max_neigh,
+                                             double *values,
+                                             const int *nlist,
+                                             const int *n_neigh,
+                                             double average_volume,
+                                             double family_volume_bulk)
+{
+    struct oracle_surface s = {NULL, n_neigh, average_volume,
+                               family_volume_bulk};
+    oracle_pairwise(nnodes, max_neigh, values, nlist, n_neigh,
+                    oracle_volume_method, &s);
+}
+
+/* correction.pyx:59-120 (set_precise_surface_correction): the family
+ * volume of i is the in-order sum of its live neighbours' volumes
+ * (:87-95), starting from 0.0.  `family_volume` is caller scratch (nnodes). */
+void oracle_set_precise_surface_correction(int nnodes, int max_neigh,
+                                           double *values, const int *nlist,
+                                           const int *n_neigh,
+                                           const double *volume,
+                                           double family_volume_bulk,
+                                           double *family_volume)
+{
+    for (int i = 0; i < nnodes; ++i) {
+        double tmp = 0.0;
+        for (int neigh = 0; neigh < n_neigh[i]; ++neigh) {
+            int j = nlist[(size_t)i * max_neigh + neigh];
+            if (j >= 0 && j < nnodes)
+                tmp += volume[j];
+        }
+        family_volume[i] = tmp;
+    }
+    struct oracle_surface s = {family_volume, n_neigh, 0.0,
+                               family_volume_bulk};
+    oracle_pairwise(nnodes, max_neigh, values, nlist, n_neigh,
+                    oracle_volume_method, &s);
+}

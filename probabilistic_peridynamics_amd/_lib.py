@@ -70,6 +70,18 @@ SIGNATURES = {
     "pd_neighbour_list": (ctypes.c_int, [
         ctypes.c_int, _D, ctypes.c_double, _I, _I, ctypes.c_int,
         ctypes.c_int]),
+    "pd_set_micromodulus_function": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I, _I,
+        ctypes.c_double, ctypes.c_int]),
+    "pd_set_volume_correction": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I, _I,
+        ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "pd_set_imprecise_surface_correction": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I,
+        ctypes.c_double, ctypes.c_double]),
+    "pd_set_precise_surface_correction": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I, _D,
+        ctypes.c_double]),
 }
 
 _lib = None

@@ -1,7 +1,9 @@
 """Build libpd_hip.so in-tree with hipcc for gfx950.
 
 ``python -m probabilistic_peridynamics_amd.build`` compiles
-``csrc/pd_hip.hip`` and ``csrc/pd_nlist.hip`` into ``probabilistic_peridynamics_amd/libpd_hip.so``.  It
+``csrc/pd_hip.hip`` (time stepping), ``csrc/pd_nlist.hip`` (neighbour list)
+and ``csrc/pd_corrections.hip`` (stiffness corrections) into
+``probabilistic_peridynamics_amd/libpd_hip.so``.  It
 needs hipcc but no GPU.
 
 Flags:
@@ -18,7 +20,8 @@ import subprocess
 import sys
 
 HERE = pathlib.Path(__file__).resolve().parent
-SOURCES = [HERE / "csrc" / "pd_hip.hip", HERE / "csrc" / "pd_nlist.hip"]
+SOURCES = [HERE / "csrc" / name for name in (
+    "pd_hip.hip", "pd_nlist.hip", "pd_corrections.hip")]
 TARGET = HERE / "libpd_hip.so"
 ARCH = os.environ.get("PD_OFFLOAD_ARCH", "gfx950")
 

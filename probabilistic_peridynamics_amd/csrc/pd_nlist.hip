@@ -11,12 +11,11 @@
 
 #include <algorithm>
 #include <cmath>
-#include <string>
 #include <vector>
 
-#include "../../include/pd_hip.h"
+#include "pd_dev.h"
 
-int pd_fail(int code, const char *msg);
+using pdsetup::Dev;
 
 namespace {
 
@@ -108,30 +107,8 @@ __global__ void k_nl_fill(int n, Grid g, const double *x,
     (void)family;
 }
 
-template <typename T>
-struct Dev {
-    T *p = nullptr;
-    ~Dev()
-    {
-        if (p)
-            (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count)
-    {
-        return hipMalloc(reinterpret_cast<void **>(&p),
-                         std::max<size_t>(count, 1) * sizeof(T));
-    }
-};
-
 }  // namespace
 
-#define NLCHK(call)                                                         \
-    do {                                                                    \
-        hipError_t e_ = (call);                                             \
-        if (e_ != hipSuccess)                                               \
-            return pd_fail(PD_E_HIP, (std::string(#call) + ": " +           \
-                                      hipGetErrorString(e_)).c_str());     \
-    } while (0)
 
 extern "C" int pd_neighbour_list_device(int device, int n, const double *x,
                                         double horizon, int *family,
@@ -144,12 +121,8 @@ extern "C" int pd_neighbour_list_device(int device, int n, const double *x,
         return pd_fail(PD_E_ARG, "horizon must be > 0");
     if (nlist && (max_neighbours <= 0 || max_neighbours > 4096))
         return pd_fail(PD_E_LIMIT, "max_neighbours must be in 1..4096");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return pd_fail(PD_E_NODEV, "no HIP device is available");
-    if (device < 0 || device >= ndev)
-        return pd_fail(PD_E_ARG, "device index out of range");
-    NLCHK(hipSetDevice(device));
+    if (int rc = pdsetup::use_device(device))
+        return rc;
 
     // cells of edge `horizon` over the bounding box, as the host search
     Grid g;
@@ -196,21 +169,21 @@ extern "C" int pd_neighbour_list_device(int device, int n, const double *x,
     Dev<double> dx;
     Dev<long long> dcs;
     Dev<int> dm, dfam;
-    NLCHK(dx.alloc(3 * (size_t)n));
-    NLCHK(dcs.alloc(cstart.size()));
-    NLCHK(dm.alloc(n));
-    NLCHK(dfam.alloc(n));
-    NLCHK(hipMemcpy(dx.p, x, 3 * (size_t)n * sizeof(double),
+    PD_SETUP_CHK(dx.alloc(3 * (size_t)n));
+    PD_SETUP_CHK(dcs.alloc(cstart.size()));
+    PD_SETUP_CHK(dm.alloc(n));
+    PD_SETUP_CHK(dfam.alloc(n));
+    PD_SETUP_CHK(hipMemcpy(dx.p, x, 3 * (size_t)n * sizeof(double),
                     hipMemcpyHostToDevice));
-    NLCHK(hipMemcpy(dcs.p, cstart.data(), cstart.size() * sizeof(long long),
+    PD_SETUP_CHK(hipMemcpy(dcs.p, cstart.data(), cstart.size() * sizeof(long long),
                     hipMemcpyHostToDevice));
-    NLCHK(hipMemcpy(dm.p, members.data(), (size_t)n * sizeof(int),
+    PD_SETUP_CHK(hipMemcpy(dm.p, members.data(), (size_t)n * sizeof(int),
                     hipMemcpyHostToDevice));
     const int blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_nl_count, dim3(blocks), dim3(256), 0, 0, n, g, dx.p,
                        dcs.p, dm.p, dfam.p);
-    NLCHK(hipGetLastError());
-    NLCHK(hipMemcpy(family, dfam.p, (size_t)n * sizeof(int),
+    PD_SETUP_CHK(hipGetLastError());
+    PD_SETUP_CHK(hipMemcpy(family, dfam.p, (size_t)n * sizeof(int),
                     hipMemcpyDeviceToHost));
     if (!nlist)
         return PD_OK;
@@ -221,12 +194,12 @@ extern "C" int pd_neighbour_list_device(int device, int n, const double *x,
     // LDS per block: threads x (K+1) ints, at most 64 KiB
     const int tpb = std::max(1, std::min(64, 16384 / (K + 1)));
     Dev<int> dnl;
-    NLCHK(dnl.alloc((size_t)n * K));
+    PD_SETUP_CHK(dnl.alloc((size_t)n * K));
     hipLaunchKernelGGL(k_nl_fill, dim3((n + tpb - 1) / tpb), dim3(tpb),
                        (size_t)tpb * (K + 1) * sizeof(int), 0, n, g, dx.p,
                        dcs.p, dm.p, dfam.p, K, pad_value, dnl.p);
-    NLCHK(hipGetLastError());
-    NLCHK(hipMemcpy(nlist, dnl.p, (size_t)n * K * sizeof(int),
+    PD_SETUP_CHK(hipGetLastError());
+    PD_SETUP_CHK(hipMemcpy(nlist, dnl.p, (size_t)n * K * sizeof(int),
                     hipMemcpyDeviceToHost));
     return PD_OK;
 }

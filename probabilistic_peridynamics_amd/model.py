@@ -13,9 +13,9 @@ Differences from the reference, all outside the time-stepping path:
   KD-tree query (distance <= horizon), rows sorted ascending -- the order of
   the reference's own fixtures (peripy/test/data/expected_connectivity*.npz);
 * model arrays given ``write_path`` are saved as .npy files (h5py is absent);
-* the stiffness-correction algorithms (surface_correction,
-  volume_correction, micromodulus_function; peripy/correction.pyx) are not
-  built: pass a precomputed ``stiffness_corrections`` array instead.
+* the stiffness-correction factors (surface_correction, volume_correction,
+  micromodulus_function) come from :mod:`.correction`, on the integrator's
+  GPU when it has one.
 """
 import pathlib
 import warnings
@@ -24,6 +24,10 @@ from collections import namedtuple
 import numpy as np
 
 from . import _lib, mesh as _mesh
+from .correction import (set_imprecise_surface_correction,
+                         set_micromodulus_function,
+                         set_precise_surface_correction,
+                         set_volume_correction)
 from .integrators import Integrator
 
 _MeshElements = namedtuple("MeshElements", ["connectivity", "boundary"])
@@ -108,12 +112,22 @@ class Model(object):
             raise TypeError("volume type is wrong (expected {}, got "
                             "{})".format(type(volume), np.ndarray))
 
-        if (surface_correction is not None or volume_correction is not None
-                or micromodulus_function is not None):
-            raise NotImplementedError(
-                "stiffness correction algorithms are not built in this "
-                "package; pass a precomputed stiffness_corrections array")
-        self.horizon = horizon
+        if volume_correction is not None:
+            if not ((type(node_radius) == float)
+                    or (type(node_radius) == np.float64)):
+                suggested = 0.5 * np.power(np.sum(self.volume) / self.nnodes,
+                                           1. / 3)
+                raise TypeError(
+                    "If volume_correction (= {}) is applied, an average "
+                    "node radius must be supplied as the keyword argument "
+                    "node_radius, a float (got {}; half the cube root of "
+                    "the mean node volume is {})".format(
+                        volume_correction, type(node_radius), suggested))
+            # partial volumes up to a node radius beyond the horizon take
+            # part in the force integral, so the search reaches that far
+            self.horizon = horizon + node_radius
+        else:
+            self.horizon = horizon
 
         if family is None or connectivity is None:
             this_may_take_a_while(self.nnodes, "family, connectivity")
@@ -171,7 +185,34 @@ class Model(object):
         self.degrees_freedom = 3
 
         if stiffness_corrections is None:
-            self.stiffness_corrections = None
+            # model.py:349-383: ones, times the micromodulus values, the
+            # volume corrections and the surface corrections, in that order
+            applied = False
+            stiffness_corrections = np.ones(
+                (self.nnodes, self.max_neighbours), dtype=np.float64)
+            if micromodulus_function is not None:
+                stiffness_corrections = self._set_micromodulus_values(
+                    micromodulus_function, stiffness_corrections, horizon)
+                applied = True
+            if volume_correction is not None:
+                # model.py:363-365 passes node_radius in the horizon position
+                # and horizon in the node_radius position; kept, so that the
+                # factors are the reference's
+                stiffness_corrections = self._set_volume_corrections(
+                    volume_correction, stiffness_corrections, node_radius,
+                    horizon)
+                applied = True
+            if surface_correction is not None:
+                stiffness_corrections = self._set_surface_corrections(
+                    surface_correction, stiffness_corrections)
+                applied = True
+            if applied:
+                self.stiffness_corrections = stiffness_corrections
+                if self.write_path is not None:
+                    write_array(self.write_path, "stiffness_corrections",
+                                stiffness_corrections)
+            else:
+                self.stiffness_corrections = None
         elif type(stiffness_corrections) == np.ndarray:
             if np.shape(stiffness_corrections) != (self.nnodes,
                                                    self.max_neighbours):
@@ -180,7 +221,12 @@ class Model(object):
                                  "(expected {}, got {})".format(
                                      (self.nnodes, self.max_neighbours),
                                      np.shape(stiffness_corrections)))
-            warnings.warn("Reading stiffness_corrections from argument.")
+            warnings.warn(
+                "Reading stiffness_corrections from argument; the flags "
+                "surface_correction={}, volume_correction={} and "
+                "micromodulus_function={} are not applied.".format(
+                    surface_correction, volume_correction,
+                    micromodulus_function))
             self.stiffness_corrections = stiffness_corrections.astype(
                 np.float64)
         else:
@@ -269,6 +315,76 @@ class Model(object):
             cells.append((self.mesh_elements.boundary, self.mesh_boundary))
         _mesh.write_vtk(filename, self.coords, cells,
                         {"damage": damage, "displacements": displacements})
+
+    def _setup_device(self):
+        """GPU for set-up work: the integrator's, if present, else -1
+        (the host)."""
+        context = self.integrator.context
+        device = (getattr(context, "device", 0) if context
+                  else getattr(self.integrator, "device", 0))
+        try:
+            device = int(device)
+        except (TypeError, ValueError):
+            return -1
+        return device if _lib.device_available(device) else -1
+
+    def _set_micromodulus_values(self, micromodulus_function,
+                                 stiffness_corrections, horizon):
+        """Multiply in the conical micromodulus values (model.py:782-821);
+        only micromodulus_function == 0 exists."""
+        nlist, n_neigh = self.initial_connectivity
+        if micromodulus_function == 0:
+            set_micromodulus_function(
+                stiffness_corrections, self.coords, nlist, n_neigh,
+                np.float64(horizon), np.intc(micromodulus_function),
+                device=self._setup_device())
+        else:
+            raise ValueError("micromodulus_function value is wrong "
+                             "(expected 0 or None, got {})".format(
+                                 micromodulus_function))
+        return stiffness_corrections
+
+    def _set_volume_corrections(self, volume_correction,
+                                stiffness_corrections, horizon, node_radius):
+        """Multiply in the partial-volume factors (model.py:823-876); only
+        volume_correction == 0 (the HHB algorithm) exists."""
+        nlist, n_neigh = self.initial_connectivity
+        if volume_correction == 0:
+            set_volume_correction(
+                stiffness_corrections, self.coords, nlist, n_neigh,
+                np.float64(horizon), np.float64(node_radius),
+                np.intc(volume_correction), device=self._setup_device())
+        else:
+            raise ValueError("volume_correction value is wrong "
+                             "(expected 0 or None, got {})".format(
+                                 volume_correction))
+        return stiffness_corrections
+
+    def _set_surface_corrections(self, surface_correction,
+                                 stiffness_corrections):
+        """Multiply in the volume-method surface factors (model.py:878-939):
+        1 uses the nodal volumes, 0 their mean.  The bulk family volume is
+        pi h^2 (2-D) or 4/3 pi h^3 (3-D) of the model's horizon."""
+        nlist, n_neigh = self.initial_connectivity
+        if self.dimensions == 2:
+            family_volume_bulk = np.float64(np.pi * np.power(self.horizon, 2))
+        else:
+            family_volume_bulk = np.float64(
+                (4. / 3) * np.pi * np.power(self.horizon, 3))
+        if surface_correction == 1:
+            set_precise_surface_correction(
+                stiffness_corrections, nlist, n_neigh, self.volume,
+                family_volume_bulk, device=self._setup_device())
+        elif surface_correction == 0:
+            average_node_volume = np.float64(np.sum(self.volume) / self.nnodes)
+            set_imprecise_surface_correction(
+                stiffness_corrections, nlist, n_neigh, average_node_volume,
+                family_volume_bulk, device=self._setup_device())
+        else:
+            raise ValueError("surface_correction value is wrong "
+                             "(expected 0, 1 or None, got {})".format(
+                                 surface_correction))
+        return stiffness_corrections
 
     def _set_neighbour_list(self, coords, horizon, nnodes,
                             initial_crack=None, context=None):

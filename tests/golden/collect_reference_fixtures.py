@@ -8,6 +8,7 @@ this script so that the tests run without the reference present.
 Each fixture and the reference test that reads it:
 
   example_mesh.vtk               conftest.py:72 (2113-node 2-D plate)
+  example_mesh_3d.vtk            test_model.py:56-88 (1175-node 3-D block)
   force_test.vtk                 test_model.py:1569 (3-node force sign model)
   expected_volume.npy            test_model.py:263
   expected_connectivity_basic.npz      test_model.py:317 (pre-crack, Cython)
@@ -19,6 +20,12 @@ Each fixture and the reference test that reads it:
   expected_*_velocity_verlet.npy test_integrators.py:721-727
   expected_bc_types/values.npy, expected_force_bc_types/values.npy
                                  test_model.py:1466-1471
+  expected_corrections.npz       the twelve expected_micromodulus_values_*,
+                                 expected_volume_corrections_* and
+                                 expected_stiffness_corrections_* arrays of
+                                 test_model.py:805-1029, packed unchanged (one
+                                 key per original file stem) into one
+                                 compressed archive
 """
 import pathlib
 import shutil
@@ -29,6 +36,7 @@ DST = pathlib.Path(__file__).resolve().parent
 
 FILES = [
     "example_mesh.vtk",
+    "example_mesh_3d.vtk",
     "force_test.vtk",
     "expected_volume.npy",
     "expected_connectivity_basic.npz",
@@ -52,9 +60,31 @@ FILES = [
     "expected_force_bc_values.npy",
 ]
 
+PACKED = {
+    "expected_corrections.npz": [
+        "expected_micromodulus_values_2d",
+        "expected_micromodulus_values_3d",
+        "expected_volume_corrections_2d",
+        "expected_volume_corrections_3d",
+        "expected_stiffness_corrections_2d",
+        "expected_stiffness_corrections_2d_cl",
+        "expected_stiffness_corrections_2d_precise",
+        "expected_stiffness_corrections_2d_precise_cl",
+        "expected_stiffness_corrections_3d",
+        "expected_stiffness_corrections_3d_cl",
+        "expected_stiffness_corrections_3d_precise",
+        "expected_stiffness_corrections_3d_precise_cl",
+    ],
+}
+
 if __name__ == "__main__":
     if not SRC.is_dir():
         sys.exit(f"{SRC} not found: the fixtures are already committed.")
     for name in FILES:
         shutil.copyfile(SRC / name, DST / name)
         print("copied", name)
+    import numpy as np
+    for archive, stems in PACKED.items():
+        np.savez_compressed(DST / archive, **{
+            stem: np.load(SRC / f"{stem}.npy") for stem in stems})
+        print("packed", archive)

@@ -35,10 +35,15 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.SIGNATURES)
 
 
-def test_library_loads_and_reports_no_error():
-    assert _lib.lib().pd_last_error() == b""
+def test_library_loads_and_reports_errors():
+    assert isinstance(_lib.lib().pd_last_error(), bytes)
     n = _lib.device_count()
     assert n >= 0
+    # a rejected call leaves its message for pd_last_error
+    rc = _lib.lib().pd_set_imprecise_surface_correction(
+        -1, 0, 0, None, None, None, 1.0, 1.0)
+    assert rc == _lib.PD_E_ARG
+    assert b"required" in _lib.lib().pd_last_error()
 
 
 def test_neighbour_list_matches_reference_sets(golden):
