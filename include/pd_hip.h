@@ -177,6 +177,22 @@ int pd_adjoint_backward(pd_sim *sim, const double *dL_du, double *dL_dc);
 /* Free the recording. */
 int pd_adjoint_stop(pd_sim *sim);
 
+/* Tip groups for pd_monitor: entry e is DOF dof[e] (0..2) of node node[e]
+ * and belongs to group group[e] (0..ngroups-1); entries keep their order
+ * inside a group.  These are the tip lists Model builds from is_tip
+ * (model.py:1093-1212).  May be called before or after pd_create_buffers;
+ * nentries == 0 clears them. */
+int pd_set_tips(pd_sim *sim, int nentries, const int *node, const int *dof,
+                const int *group, int ngroups);
+
+/* The sums one periodic write of Model.simulate records (model.py:1303-1360),
+ * computed on the device instead of after a full pd_write: model[6] = sum of
+ * u, ud, udd, force*V and body_force*V over all DOFs, and sum of damage;
+ * tips[5*g .. 5*g+4] = the same five sums over tip group g, accumulated from
+ * 0.0 in entry order as the reference's loop does (its values bitwise).  The
+ * model sums use a fixed reduction order (not numpy's).  Synchronizes. */
+int pd_monitor(pd_sim *sim, double *model, double *tips);
+
 /* Integrator.write (integrators.py:285-300): damage, then device -> host
  * copies.  Any pointer may be NULL to skip that array.  nlist (N,K) and
  * n_neigh (N) come back in the layout given to pd_build, reflecting every
@@ -270,6 +286,30 @@ int pd_set_precise_surface_correction(int device, int nnodes,
                                       const int *nlist, const int *n_neigh,
                                       const double *volume,
                                       double family_volume_bulk);
+
+/* Initial crack: the bonds (i, j), i < j, that cross a straight crack, in
+ * the order initial_crack_helper lists them (model.py:1578-1612: i
+ * ascending, then i's row order), found on `device` (-1: the host).  The
+ * crack is the bulk form of the reference's pairwise predicates
+ * (peripy/test/conftest.py:43-63): with p1 the end of smaller coordinate
+ * `axis` and p2 the other, a bond crosses when p1[axis] < split < p2[axis]
+ * and the line through p1 and p2 meets coordinate `axis` = at at a
+ * `cross_axis` coordinate strictly between lo and hi, computed as
+ * m = (p2[c]-p1[c])/(p2[a]-p1[a]); b = p1[c] - m*p1[a]; m*at + b.
+ * *npairs receives the number of pairs; pairs (capacity rows of 2 ints) is
+ * filled when given and large enough (PD_E_LIMIT otherwise). */
+int pd_line_crack_pairs(int device, int nnodes, int max_neighbours,
+                        const double *coords, const int *nlist,
+                        const int *n_neigh, int axis, int cross_axis,
+                        double split, double at, double lo, double hi,
+                        int *pairs, long long capacity, long long *npairs);
+
+/* create_crack.pyx:1-43, on the host: for each pair (i, j) in order, remove
+ * j from row i and then i from row j -- the first slot holding it takes the
+ * row's last live entry, that slot becomes -1, n_neigh decreases.  Every
+ * pair must name nodes in [0, nnodes) (checked before anything changes). */
+int pd_create_crack(int nnodes, int max_neighbours, long long ncrack,
+                    const int *crack, int *nlist, int *n_neigh);
 
 #ifdef __cplusplus
 }

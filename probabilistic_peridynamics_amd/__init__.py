@@ -11,12 +11,13 @@ from .integrators import (Integrator, Euler, EulerHIP, EulerCromerHIP,
                           get_context)
 from .model import (Model, initial_crack_helper, DimensionalityError,
                     FamilyError, DamageModelError, InvalidIntegrator)
+from .crack import LineCrack
 
 __all__ = [
     "Model", "initial_crack_helper", "Integrator", "Euler", "EulerHIP",
     "EulerCromerHIP", "VelocityVerletHIP", "EulerCL", "EulerCromerCL",
     "VelocityVerletCL", "HIPContext", "ContextError", "get_context",
     "DimensionalityError", "FamilyError", "DamageModelError",
-    "InvalidIntegrator",
+    "InvalidIntegrator", "LineCrack",
 ]
 __version__ = "0.1.0"

@@ -56,6 +56,9 @@ SIGNATURES = {
     "pd_run": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D]),
     "pd_run_timed": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D, _F]),
     "pd_write": (ctypes.c_int, [_SIM, _D, _D, _D, _D, _D, _D, _I, _I]),
+    "pd_set_tips": (ctypes.c_int, [_SIM, ctypes.c_int, _I, _I, _I,
+                                   ctypes.c_int]),
+    "pd_monitor": (ctypes.c_int, [_SIM, _D, _D]),
     "pd_get_regimes": (ctypes.c_int, [_SIM, _I]),
     "pd_live_bonds": (ctypes.c_int, [_SIM, _LL]),
     "pd_synchronize": (ctypes.c_int, [_SIM]),
@@ -82,6 +85,12 @@ SIGNATURES = {
     "pd_set_precise_surface_correction": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I, _D,
         ctypes.c_double]),
+    "pd_line_crack_pairs": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I, ctypes.c_int,
+        ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+        ctypes.c_double, _I, ctypes.c_longlong, _LL]),
+    "pd_create_crack": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _I, _I, _I]),
 }
 
 _lib = None

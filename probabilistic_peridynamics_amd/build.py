@@ -1,8 +1,9 @@
 """Build libpd_hip.so in-tree with hipcc for gfx950.
 
 ``python -m probabilistic_peridynamics_amd.build`` compiles
-``csrc/pd_hip.hip`` (time stepping), ``csrc/pd_nlist.hip`` (neighbour list)
-and ``csrc/pd_corrections.hip`` (stiffness corrections) into
+``csrc/pd_hip.hip`` (time stepping), ``csrc/pd_nlist.hip`` (neighbour list),
+``csrc/pd_corrections.hip`` (stiffness corrections) and ``csrc/pd_crack.hip``
+(initial crack) into
 ``probabilistic_peridynamics_amd/libpd_hip.so``.  It
 needs hipcc but no GPU.
 
@@ -21,7 +22,7 @@ import sys
 
 HERE = pathlib.Path(__file__).resolve().parent
 SOURCES = [HERE / "csrc" / name for name in (
-    "pd_hip.hip", "pd_nlist.hip", "pd_corrections.hip")]
+    "pd_hip.hip", "pd_nlist.hip", "pd_corrections.hip", "pd_crack.hip")]
 TARGET = HERE / "libpd_hip.so"
 ARCH = os.environ.get("PD_OFFLOAD_ARCH", "gfx950")
 

@@ -712,7 +712,8 @@ __global__ void k_records(int n, double wscale, const double *r0,
 // (peridynamics.pyx:29), and, for the slot layout, the reference-layout
 // neighbour list: -1 everywhere but at the live slots' columns.  Thread p
 // handles internal node p and writes row tnode[p] of the outputs.
-__global__ void k_export(int n, int K, int layout, const int *tnode,
+// rows == 0 counts the live bonds without rebuilding the slot-layout rows.
+__global__ void k_export(int n, int K, int layout, int rows, const int *tnode,
                          const int *cnt, const long long *chunk0,
                          const int *stride, const int *gidx,
                          const unsigned short *pos, const int *family,
@@ -726,15 +727,17 @@ __global__ void k_export(int n, int K, int layout, const int *tnode,
     int live;
     if (layout == PD_LAYOUT_SLOT) {
         int *row = nlist_out + (size_t)i * K;
-        for (int q = 0; q < K; ++q)
-            row[q] = -1;
+        if (rows)
+            for (int q = 0; q < K; ++q)
+                row[q] = -1;
         live = 0;
         const int c = cnt[p];
         for (int k = 0; k < c; ++k) {
             const size_t s = slot_of(chunk0[p], stride[p], k);
             const int e = gidx[s];
             if (e >= 0) {
-                row[pos[s]] = e;
+                if (rows)
+                    row[pos[s]] = e;
                 ++live;
             }
         }
@@ -744,6 +747,80 @@ __global__ void k_export(int n, int K, int layout, const int *tnode,
     n_neigh_out[i] = live;
     const double f = (double)family[i];
     damage[i] = (f - (double)live) / f;
+}
+
+// Monitored sums of a write (model.py:1320-1360).  Pass 1: block b sums a
+// fixed strided share of the DOFs -- u, ud, udd, force*V, body_force*V --
+// and of the nodes' damage, then reduces its 256 lanes by a fixed tree, so
+// the result does not depend on scheduling.
+constexpr int MON_BLOCKS = 512;
+constexpr int MON_SUMS = 6;
+
+__global__ void __launch_bounds__(256)
+    k_monitor_part(int n, const double *u, const double *ud,
+                   const double *udd, const double *force,
+                   const double *body, const double *vol,
+                   const double *damage, double *part)
+{
+    __shared__ double sh[MON_SUMS][256];
+    const int t = threadIdx.x;
+    const size_t first = (size_t)blockIdx.x * 256 + t;
+    const size_t step = (size_t)MON_BLOCKS * 256;
+    double a[MON_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (size_t q = first; q < 3 * (size_t)n; q += step) {
+        const double v = vol[q / 3];
+        a[0] += u[q];
+        a[1] += ud[q];
+        a[2] += udd[q];
+        a[3] += force[q] * v;
+        a[4] += body[q] * v;
+    }
+    for (size_t i = first; i < (size_t)n; i += step)
+        a[5] += damage[i];
+    for (int k = 0; k < MON_SUMS; ++k)
+        sh[k][t] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < MON_SUMS; ++k)
+                sh[k][t] += sh[k][t + w];
+        __syncthreads();
+    }
+    if (t < MON_SUMS)
+        part[(size_t)blockIdx.x * MON_SUMS + t] = sh[t][0];
+}
+
+// Pass 2 (one block): the block sums in block order, and each tip group's
+// sums accumulated entry by entry from 0.0 in the order the entries were
+// given -- the reference's loop over the group's (node, dof) list
+// (model.py:1320-1336), so these are its values bitwise.
+__global__ void __launch_bounds__(256)
+    k_monitor_final(const double *part, int ngroups, const int *tip_off,
+                    const int *tip_q, const double *u, const double *ud,
+                    const double *udd, const double *force,
+                    const double *body, const double *vol, double *out)
+{
+    const int t = threadIdx.x;
+    if (t < MON_SUMS) {
+        double acc = 0.0;
+        for (int b = 0; b < MON_BLOCKS; ++b)
+            acc += part[b * MON_SUMS + t];
+        out[t] = acc;
+    }
+    for (int g = t; g < ngroups; g += blockDim.x) {
+        double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int e = tip_off[g]; e < tip_off[g + 1]; ++e) {
+            const int q = tip_q[e];
+            const double v = vol[q / 3];
+            a[0] += u[q];
+            a[1] += ud[q];
+            a[2] += udd[q];
+            a[3] += force[q] * v;
+            a[4] += body[q] * v;
+        }
+        for (int k = 0; k < 5; ++k)
+            out[MON_SUMS + 5 * g + k] = a[k];
+    }
 }
 
 __global__ void k_export_regimes(int n, int K, const int *tnode,
@@ -1002,6 +1079,11 @@ struct pd_sim {
     DevBuf<int> brk, jint;
     DevBuf<unsigned short> loc0;
     DevBuf<double> ckpt, seg, lam[2], gradc, fin_u, fin_force, fin_bforce;
+
+    // monitored sums: tip entries (caller's node, dof) grouped, in order
+    std::vector<int> h_tip_node, h_tip_dof, h_tip_off;
+    DevBuf<int> tip_q, tip_off;  // internal DOF index 3p + d; group offsets
+    DevBuf<double> mon_part, mon_out;
 };
 
 namespace {
@@ -1233,6 +1315,22 @@ int launch_step(pd_sim *s, double dmag, double fmag)
                                   hipGetErrorString(e));
     s->cur = 1 - s->cur;
     return after_step(s, dmag, fmag);
+}
+
+// The tip entries as internal DOF indices 3p + d for the current tiling.
+int upload_tips(pd_sim *s)
+{
+    if (s->h_tip_off.empty() || s->perm.empty())
+        return PD_OK;
+    std::vector<int> inv(s->n);
+    for (int p = 0; p < s->n; ++p)
+        inv[s->perm[p]] = p;
+    std::vector<int> q(s->h_tip_node.size());
+    for (size_t e = 0; e < q.size(); ++e)
+        q[e] = 3 * inv[s->h_tip_node[e]] + s->h_tip_dof[e];
+    HIPCHK(upload(s->tip_q, q.data(), q.size()));
+    HIPCHK(upload(s->tip_off, s->h_tip_off.data(), s->h_tip_off.size()));
+    return PD_OK;
 }
 
 // (Re)write both record buffers from r0, V, the device u and c.
@@ -1799,9 +1897,94 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     rc = refresh_records(s);
     if (rc)
         return rc;
+    rc = upload_tips(s);
+    if (rc)
+        return rc;
     s->has_buffers = true;
     s->retile_pending = false;
     adjoint_release(s);  // slot arrays were rebuilt
+    return PD_OK;
+}
+
+int pd_set_tips(pd_sim *s, int nentries, const int *node, const int *dof,
+                const int *group, int ngroups)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (nentries < 0 || ngroups < 0 ||
+        (nentries > 0 && (!node || !dof || !group)))
+        return fail(PD_E_ARG, "node, dof and group (nentries) are required");
+    std::vector<int> off(ngroups + 1, 0);
+    for (int e = 0; e < nentries; ++e) {
+        if (node[e] < 0 || node[e] >= s->n || dof[e] < 0 || dof[e] > 2 ||
+            group[e] < 0 || group[e] >= ngroups)
+            return fail(PD_E_ARG, "a tip entry is out of range");
+        off[group[e] + 1]++;
+    }
+    for (int g = 0; g < ngroups; ++g)
+        off[g + 1] += off[g];
+    // group the entries, keeping their order inside a group
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    s->h_tip_node.assign(nentries, 0);
+    s->h_tip_dof.assign(nentries, 0);
+    for (int e = 0; e < nentries; ++e) {
+        const int at = fill[group[e]]++;
+        s->h_tip_node[at] = node[e];
+        s->h_tip_dof[at] = dof[e];
+    }
+    s->h_tip_off = off;
+    s->tip_q.release();
+    s->tip_off.release();
+    if (s->has_buffers)
+        return upload_tips(s);
+    return PD_OK;
+}
+
+int pd_monitor(pd_sim *s, double *model, double *tips)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!s->has_buffers)
+        return fail(PD_E_STATE, "pd_create_buffers has not been called");
+    if (!model)
+        return fail(PD_E_ARG, "model (6 doubles) is required");
+    const int ngroups =
+        s->h_tip_off.empty() ? 0 : (int)s->h_tip_off.size() - 1;
+    if (ngroups > 0 && !tips)
+        return fail(PD_E_ARG, "tips (5 doubles per group) is required");
+    const int N = s->n;
+    if (!s->mon_part.p) {
+        HIPCHK(s->mon_part.alloc((size_t)MON_BLOCKS * MON_SUMS));
+    }
+    const size_t nout = MON_SUMS + 5 * (size_t)ngroups;
+    if (s->mon_out.n < nout) {
+        HIPCHK(s->mon_out.alloc(nout));
+    }
+    hipLaunchKernelGGL(k_export, dim3((N + 255) / 256), dim3(256), 0,
+                       s->stream, N, s->K, s->layout, 0, s->tnode.p,
+                       s->cnt.p, s->chunk0.p, s->stride.p, s->gidx.p,
+                       s->pos.p, s->family.p, s->n_neigh.p, s->out_nlist.p,
+                       s->out_nneigh.p, s->damage.p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_monitor_part, dim3(MON_BLOCKS), dim3(256), 0,
+                       s->stream, N, s->u.p, s->ud.p, s->udd.p, s->force.p,
+                       s->body_force.p, s->vol.p, s->damage.p,
+                       s->mon_part.p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s->stream,
+                       s->mon_part.p, ngroups, s->tip_off.p, s->tip_q.p,
+                       s->u.p, s->ud.p, s->udd.p, s->force.p,
+                       s->body_force.p, s->vol.p, s->mon_out.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(nout);
+    HIPCHK(hipMemcpyAsync(h.data(), s->mon_out.p, nout * sizeof(double),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::copy(h.begin(), h.begin() + MON_SUMS, model);
+    if (ngroups > 0)
+        std::copy(h.begin() + MON_SUMS, h.end(), tips);
     return PD_OK;
 }
 
@@ -2154,7 +2337,8 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
     const int N = s->n;
     const int blocks = (N + 255) / 256;
     hipLaunchKernelGGL(k_export, dim3(blocks), dim3(256), 0, s->stream, N,
-                       s->K, s->layout, s->tnode.p, s->cnt.p, s->chunk0.p,
+                       s->K, s->layout, nlist ? 1 : 0, s->tnode.p, s->cnt.p,
+                       s->chunk0.p,
                        s->stride.p, s->gidx.p, s->pos.p, s->family.p,
                        s->n_neigh.p, s->out_nlist.p, s->out_nneigh.p,
                        s->damage.p);

@@ -231,6 +231,23 @@ class Integrator(ABC):
         return (u, ud, udd, force, body_force, damage, nlist, n_neigh)
 
     # -- extensions ----------------------------------------------------
+    def write_fields(self, u, damage):
+        """:meth:`write` of u and damage only (no neighbour-list export or
+        copy): what a periodic mesh file needs."""
+        self._need_sim()
+        n = self.nnodes
+        u = u if (isinstance(u, np.ndarray) and u.shape == (n, 3)
+                  and u.dtype == np.float64 and u.flags["C_CONTIGUOUS"]
+                  and u.flags["WRITEABLE"]) else np.empty((n, 3))
+        damage = damage if (isinstance(damage, np.ndarray)
+                            and damage.shape == (n,)
+                            and damage.dtype == np.float64
+                            and damage.flags["C_CONTIGUOUS"]
+                            and damage.flags["WRITEABLE"]) else np.empty(n)
+        check(_lib.lib().pd_write(self._sim, dptr(u), None, None, None, None,
+                                  dptr(damage), None, None))
+        return u, damage
+
     def run(self, steps, displacement_bc_magnitudes=None,
             force_bc_magnitudes=None):
         """``steps`` calls of ``__call__`` in one library call."""
@@ -291,6 +308,41 @@ class Integrator(ABC):
         g = (None if group is None
              else _lib.as_i32(group, (self.nnodes,), "group"))
         check(_lib.lib().pd_set_groups(self._sim, iptr(g)))
+
+    def set_tips(self, tip_types):
+        """Tip groups for :meth:`monitor`: ``{name: [(node, dof), ...]}``
+        (the ``tip_types`` of model.py:1093-1212), order kept."""
+        self._need_sim()
+        self._tip_names = list(tip_types)
+        node, dof, group = [], [], []
+        for g, name in enumerate(self._tip_names):
+            for i, j in tip_types[name]:
+                node.append(i)
+                dof.append(j)
+                group.append(g)
+        arr = [np.ascontiguousarray(x, dtype=np.int32)
+               for x in (node, dof, group)]
+        check(_lib.lib().pd_set_tips(self._sim, len(node), *map(iptr, arr),
+                                     len(self._tip_names)))
+
+    _SUMS = ("displacement", "velocity", "acceleration", "force",
+             "body_force")
+
+    def monitor(self):
+        """The sums one periodic write records, computed on the device:
+        ``(model, tips)`` with ``model`` = {displacement, velocity,
+        acceleration, force, body_force: sums of u, ud, udd, force*V,
+        body_force*V over all DOFs; damage_sum} and ``tips`` = {tip name:
+        the same five sums over its (node, dof) entries}."""
+        self._need_sim()
+        names = getattr(self, "_tip_names", [])
+        model = np.zeros(6)
+        tips = np.zeros(5 * max(len(names), 1))
+        check(_lib.lib().pd_monitor(self._sim, dptr(model), dptr(tips)))
+        out = dict(zip(self._SUMS, model[:5]))
+        out["damage_sum"] = model[5]
+        return out, {name: dict(zip(self._SUMS, tips[5 * g:5 * g + 5]))
+                     for g, name in enumerate(names)}
 
     def synchronize(self):
         self._need_sim()

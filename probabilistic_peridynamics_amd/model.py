@@ -10,8 +10,11 @@ Differences from the reference, all outside the time-stepping path:
   and ``mesh_file`` may also be an (nnodes, 3) array of coordinates;
 * the neighbour list comes from libpd_hip's cell-list search
   (``pd_neighbour_list``): the same neighbour sets as the reference's
-  KD-tree query (distance <= horizon), rows sorted ascending -- the order of
-  the reference's own fixtures (peripy/test/data/expected_connectivity*.npz);
+  KD-tree query (distance <= horizon), rows sorted ascending.  The
+  reference's rows come in its KD-tree's internal order, which its fixtures
+  (peripy/test/data/expected_connectivity*.npz) record and which today's
+  scikit-learn no longer reproduces either; everything downstream is
+  defined per bond, so only the slot order differs;
 * model arrays given ``write_path`` are saved as .npy files (h5py is absent);
 * the stiffness-correction factors (surface_correction, volume_correction,
   micromodulus_function) come from :mod:`.correction`, on the integrator's
@@ -23,7 +26,7 @@ from collections import namedtuple
 
 import numpy as np
 
-from . import _lib, mesh as _mesh
+from . import _lib, crack as _crack, mesh as _mesh
 from .correction import (set_imprecise_surface_correction,
                          set_micromodulus_function,
                          set_precise_surface_correction,
@@ -49,22 +52,11 @@ def read_array(read_path, dataset):
 
 
 def create_crack(crack, nlist, n_neigh):
-    """Remove bonds (create_crack.pyx:1-43), in place.
-
-    For each pair (i, j): j is removed from i's row by swapping in the last
-    live entry, the vacated last slot is set to -1 and n_neigh[i] decreases;
-    then the same for i in j's row.
-    """
-    crack = np.asarray(crack, dtype=np.int64).reshape(-1, 2)
-    for i, j in crack:
-        for a, b in ((i, j), (j, i)):
-            n = n_neigh[a]
-            hit = np.nonzero(nlist[a, :n] == b)[0]
-            if hit.size:
-                p = hit[0]
-                nlist[a, p] = nlist[a, n - 1]
-                nlist[a, n - 1] = -1
-                n_neigh[a] = n - 1
+    """Remove bonds (create_crack.pyx:1-43), in place: for each pair (i, j),
+    j leaves row i by swapping in the last live entry, the vacated last slot
+    is set to -1 and n_neigh[i] decreases; then the same for i in row j.
+    Runs in libpd_hip (``pd_create_crack``); nlist and n_neigh are int32."""
+    _crack.create_crack(crack, nlist, n_neigh)
 
 
 class Model(object):
@@ -688,10 +680,35 @@ class Model(object):
             displacement_bc_magnitudes, force_bc_magnitudes, connectivity,
             bond_stiffness, critical_stretch, write_path)
 
+        # HIP integrators sum on the device and copy only u and damage (for
+        # the mesh file) at a write; other integrators take the full write
+        monitor = getattr(self.integrator, "monitor", None)
+        if write and monitor is not None:
+            self.integrator.set_tips(self.tip_types)
         for step in range(first_step, first_step + steps):
             self.integrator(displacement_bc_magnitudes[step - 1],
                             force_bc_magnitudes[step - 1])
-            if write and step % write == 0:
+            if write and step % write == 0 and monitor is not None:
+                u, damage = self.integrator.write_fields(u, damage)
+                self.write_mesh(write_path / f"U_{step}.vtk", damage, u)
+                ii = step // write - (first_step - 1) // write - 1
+                sums, tip_sums = monitor()
+                for tip_type in self.tip_types:
+                    if tip_type not in data:
+                        data[tip_type] = {
+                            k: np.zeros(nwrites, dtype=np.float64)
+                            for k in ("displacement", "velocity",
+                                      "acceleration", "force", "body_force")}
+                    for k, v in tip_sums[tip_type].items():
+                        data[tip_type][k][ii] += v
+                data["model"]["step"][ii] = step
+                for k in ("displacement", "velocity", "acceleration", "force",
+                          "body_force", "damage_sum"):
+                    data["model"][k][ii] = sums[k]
+                if sums["damage_sum"] > 0.05 * self.nnodes:
+                    warnings.warn("Over 5% of bonds have broken! "
+                                  "peridynamics simulation continuing")
+            elif write and step % write == 0:
                 (u, ud, udd, force, body_force, damage, nlist,
                  n_neigh) = self.integrator.write(
                     u, ud, udd, force, body_force, damage, nlist, n_neigh)
