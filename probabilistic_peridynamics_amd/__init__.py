@@ -9,8 +9,9 @@ from .integrators import (Integrator, Euler, EulerHIP, EulerCromerHIP,
                           VelocityVerletHIP, EulerCL, EulerCromerCL,
                           VelocityVerletCL, HIPContext, ContextError,
                           get_context)
-from .model import (Model, initial_crack_helper, DimensionalityError,
-                    FamilyError, DamageModelError, InvalidIntegrator)
+from .model import (Model, initial_crack_helper, vectorised,
+                    DimensionalityError, FamilyError, DamageModelError,
+                    InvalidIntegrator)
 from .crack import LineCrack
 
 __all__ = [
@@ -18,6 +19,6 @@ __all__ = [
     "EulerCromerHIP", "VelocityVerletHIP", "EulerCL", "EulerCromerCL",
     "VelocityVerletCL", "HIPContext", "ContextError", "get_context",
     "DimensionalityError", "FamilyError", "DamageModelError",
-    "InvalidIntegrator", "LineCrack",
+    "InvalidIntegrator", "LineCrack", "vectorised",
 ]
 __version__ = "0.1.0"

@@ -51,6 +51,28 @@ def read_array(read_path, dataset):
     return np.load(f) if f.exists() else None
 
 
+def vectorised(fn):
+    """Mark a set-up callback (``is_bond_type``, ``is_density``) as one that
+    works on many nodes at once.
+
+    The reference calls ``is_bond_type(x, y)`` once per bond and
+    ``is_density(x)`` once per node from Python (model.py:657-780): 1e8
+    calls on a million-node mesh.  A function marked with this decorator is
+    called instead with coordinate arrays of shape (3, M) -- component
+    first, so ``x[0]`` is the x coordinate of every point, as for a single
+    node -- and must return an array of M results: integer bond types for
+    ``is_bond_type``, float densities for ``is_density``.  Bonds are passed
+    in the reference's order (node i ascending, then i's row order) in
+    blocks of at most 4M bonds.  The range checks are the reference's.
+    """
+    fn.vectorised = True
+    return fn
+
+
+def _is_vectorised(fn):
+    return getattr(fn, "vectorised", False) is True
+
+
 def create_crack(crack, nlist, n_neigh):
     """Remove bonds (create_crack.pyx:1-43), in place: for each pair (i, j),
     j leaves row i by swapping in the last live entry, the vacated last slot
@@ -444,6 +466,18 @@ class Model(object):
                 return None
             if not callable(is_density):
                 raise TypeError("is_density must be a *function*.")
+            if _is_vectorised(is_density):
+                density = np.asarray(is_density(self.coords.T.copy()))
+                if density.shape != (self.nnodes,) or density.dtype.kind != "f":
+                    raise TypeError(
+                        "a vectorised is_density must return a float array of "
+                        "shape (nnodes,) (got {} of shape {})".format(
+                            density.dtype, density.shape))
+                density = density.astype(np.float64)
+                if self.write_path is not None:
+                    write_array(self.write_path, "density", density)
+                return np.transpose(np.tile(density, (self.degrees_freedom, 1))
+                                    ).astype(np.float64)
             if type(is_density(self.coords[0])) is not float:
                 raise TypeError(
                     "is_density must be a function that returns a *float* "
@@ -475,6 +509,12 @@ class Model(object):
             nlist, n_neigh = connectivity
             bond_types = np.zeros((self.nnodes, self.max_neighbours),
                                   dtype=np.intc)
+            if _is_vectorised(is_bond_type):
+                self._bond_types_vectorised(is_bond_type, nlist, n_neigh,
+                                            nbond_types, bond_types)
+                if self.write_path is not None:
+                    write_array(self.write_path, "bond_types", bond_types)
+                return bond_types
             for i in range(self.nnodes):
                 for neigh in range(n_neigh[i]):
                     j = nlist[i][neigh]
@@ -503,6 +543,46 @@ class Model(object):
         if self.write_path is not None:
             write_array(self.write_path, "bond_types", bond_types)
         return bond_types
+
+    def _bond_types_vectorised(self, is_bond_type, nlist, n_neigh,
+                               nbond_types, out, block=1 << 22):
+        """Bond types from a :func:`vectorised` is_bond_type, written into
+        ``out`` row by row in blocks of at most ``block`` bonds."""
+        nlist = np.asarray(nlist)
+        n_neigh = np.asarray(n_neigh)
+        cum = np.concatenate([[0], np.cumsum(n_neigh, dtype=np.int64)])
+        lo = 0
+        while lo < self.nnodes:
+            hi = int(np.searchsorted(cum, cum[lo] + block, side="right")) - 1
+            hi = min(max(hi, lo + 1), self.nnodes)
+            rows = nlist[lo:hi]
+            live = np.arange(self.max_neighbours)[None, :] < n_neigh[lo:hi,
+                                                                     None]
+            i = np.repeat(np.arange(lo, hi), n_neigh[lo:hi])
+            j = rows[live]
+            # component-major views: x[0] is every bond's x coordinate
+            bt = np.asarray(is_bond_type(self.coords[i].T,
+                                         self.coords[j].T))
+            if bt.shape != (len(i),) or bt.dtype.kind not in "iu":
+                raise TypeError(
+                    "a vectorised is_bond_type must return an integer array "
+                    "with one entry per bond (got {} of shape {} for {} "
+                    "bonds)".format(bt.dtype, bt.shape, len(i)))
+            bad = np.nonzero((bt < 0) | (bt > nbond_types - 1))[0]
+            if bad.size:
+                q = bad[0]
+                if bt[q] < 0:
+                    raise ValueError(
+                        "is_bond_type must be a function that returns "
+                        "a *positive* int or 0 (got {})".format(bt[q]))
+                raise ValueError(
+                    "is_bond_type must be a function that returns "
+                    "a positive int or 0 which is *less* than "
+                    "nbond_types (nbond_types = {}, got {})".format(
+                        nbond_types, bt[q]))
+            sub = out[lo:hi]
+            sub[live] = bt
+            lo = hi
 
     def _set_damage_model(self, bond_stiffness, critical_stretch):
         """Damage-model tables and plus_cs (model.py:941-1091).

@@ -26,6 +26,8 @@ Each fixture and the reference test that reads it:
                                  test_model.py:805-1029, packed unchanged (one
                                  key per original file stem) into one
                                  compressed archive
+  expected_bond_types.npz        expected_bond_types(_cl).npy of
+                                 test_model.py:543-561, packed unchanged
 """
 import pathlib
 import shutil
@@ -74,6 +76,10 @@ PACKED = {
         "expected_stiffness_corrections_3d_cl",
         "expected_stiffness_corrections_3d_precise",
         "expected_stiffness_corrections_3d_precise_cl",
+    ],
+    "expected_bond_types.npz": [
+        "expected_bond_types",
+        "expected_bond_types_cl",
     ],
 }
 
