@@ -274,3 +274,31 @@ def test_integrator_contexts():
         it.EulerHIP(1.0, context=5)
     with pytest.raises(TypeError):
         it.Integrator(1.0)
+
+
+def test_build_rejects_bad_arguments_before_touching_a_device():
+    """pd_build's argument checks (family > 0, width, densities for the
+    integrator) come before any device call, so they hold without a GPU."""
+    import ctypes
+    n, k = 3, 4
+    coords = np.zeros((n, 3))
+    vol = np.ones(n)
+    bct = np.zeros((n, 3), np.int32)
+    bcv = np.zeros((n, 3))
+
+    def build(family, kk=k, integrator=0, dens=None):
+        sim = ctypes.c_void_p()
+        return _lib.lib().pd_build(
+            ctypes.byref(sim), 0, integrator, 1, 1e-3, 0.0, n, kk,
+            _lib.dptr(coords), _lib.dptr(vol), _lib.iptr(family),
+            _lib.iptr(bct), _lib.dptr(bcv), _lib.iptr(bct), _lib.dptr(bcv),
+            None, None, _lib.dptr(dens))
+
+    fam = np.array([1, 0, 2], np.int32)
+    assert build(fam) == _lib.PD_E_ARG
+    assert b"family" in _lib.lib().pd_last_error()
+    ok = np.array([1, 1, 2], np.int32)
+    assert build(ok, kk=0) == _lib.PD_E_LIMIT
+    assert build(ok, dens=np.ones((n, 3))) == _lib.PD_E_ARG
+    assert b"densities" in _lib.lib().pd_last_error()
+    assert build(ok, integrator=2) == _lib.PD_E_ARG
