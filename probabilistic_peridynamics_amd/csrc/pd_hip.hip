@@ -297,8 +297,12 @@ constexpr unsigned piece_bytes()
 
 // W bonds of node i at slots s0 .. s0+W-1: ev their halo indices (LOC_DEAD:
 // broken or empty), st their (bond type << 8 | regime) for DM_NLIN, ci node
-// i's stiffness for DM_FIELD.
-template <int DM, int W>
+// i's stiffness for DM_FIELD.  Two compile-time switches, so that the
+// common runs spend no instruction on what they do not use: CORR, the
+// per-slot stiffness corrections multiply the force (bond_force2/4); TENS,
+// a PMB bond breaks on !(s < s0) (cl/peridynamics.cl:78) instead of
+// !(|s| < s0) (peridynamics.pyx:133).
+template <int DM, int W, bool CORR, bool TENS>
 __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
                                       const double4 &r0i, const double4 &ri,
                                       double ci,
@@ -347,7 +351,7 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
         const bool fast = core_range(qq) & core_range(q0);
         bool near;
         if (DM != DM_NLIN) {
-            near = fabs(sign_mask(sh[q], a.absmask) - a.s0h) <= a.s0h_guard;
+            near = fabs((TENS ? sh[q] : fabs(sh[q])) - a.s0h) <= a.s0h_guard;
         } else {
             const int base = (st[q] >> 8) * a.nregimes;
             const int reg = st[q] & 0xff;
@@ -380,14 +384,14 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
     double cr[W];
 #pragma unroll
     for (int q = 0; q < W; ++q)
-        cr[q] = a.has_corr ? a.corr[s0 + q] : 1.0;
+        cr[q] = CORR ? a.corr[s0 + q] : 1.0;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         if (!live[q])
             continue;
         double g;
         if (DM != DM_NLIN) {
-            if (!(sign_mask(sh[q], a.absmask) < a.s0h)) {
+            if (!((TENS ? sh[q] : fabs(sh[q])) < a.s0h)) {
                 a.loc[s0 + q] = LOC_DEAD;
                 if (a.brk)
                     a.brk[s0 + q] = a.step;
@@ -426,7 +430,8 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
                 __builtin_fma(sh[q] + sh[q], tk[base + nreg], tp[base + nreg]);
             g = fn * (hl[q] + hl[q]) * w[q];
         }
-        g *= cr[q];
+        if (CORR)
+            g *= cr[q];
         fx = __builtin_fma(g, dx[q], fx);
         fy = __builtin_fma(g, dy[q], fy);
         fz = __builtin_fma(g, dz[q], fz);
@@ -448,7 +453,7 @@ struct Chunk {
 // field and n-linear models would spill.)  Only the exact-division path -- a stretch
 // near a threshold, or a square root outside the core's range -- and
 // breaking branch.
-template <int DM>
+template <int DM, bool CORR, bool TENS>
 __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
                                        const double4 &r0i, const double4 &ri,
                                        double ci,
@@ -459,17 +464,17 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
 {
     const unsigned ev[4] = {ch.loc.x, ch.loc.y, ch.loc.z, ch.loc.w};
     const unsigned short st[4] = {ch.st.x, ch.st.y, ch.st.z, ch.st.w};
-    bonds<DM, 2>(a, ldsb, r0i, ri, ci, ev, st, s0, ts, tk, tp, fx, fy, fz,
-                 nbroken);
-    bonds<DM, 2>(a, ldsb, r0i, ri, ci, ev + 2, st + 2, s0 + 2, ts, tk, tp, fx,
-                 fy, fz, nbroken);
+    bonds<DM, 2, CORR, TENS>(a, ldsb, r0i, ri, ci, ev, st, s0, ts, tk, tp,
+                             fx, fy, fz, nbroken);
+    bonds<DM, 2, CORR, TENS>(a, ldsb, r0i, ri, ci, ev + 2, st + 2, s0 + 2,
+                             ts, tk, tp, fx, fy, fz, nbroken);
 }
 
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
 // node taking its 4-bond chunks round-robin and reducing with cross-lane
 // shuffles.  LDS (static, LDS_BYTES): the halo, four pieces of
 // piece_bytes<DM>() each, then for DM_NLIN the three damage-model tables.
-template <int DM, int LPN>
+template <int DM, int LPN, bool CORR, bool TENS>
 __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
                                                double fmag)
 {
@@ -575,8 +580,8 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         for (int c = sub; c < nch; c += LPN) {
             const Chunk nx = chunk_at(c + DEPTH * LPN);
             const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
-            chunk4<DM>(a, ldsb, r0i, ri, ci, q0, s0, ts, tk, tp, fx, fy, fz,
-                       nbroken);
+            chunk4<DM, CORR, TENS>(a, ldsb, r0i, ri, ci, q0, s0, ts, tk, tp,
+                                   fx, fy, fz, nbroken);
             q0 = q1;
             q1 = q2;
             q2 = q3;
@@ -1222,8 +1227,21 @@ hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
 {
     const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
     const unsigned block = (unsigned)(s->tile_nodes * LPN);
-    hipLaunchKernelGGL((k_step<DM, LPN>), dim3(grid), dim3(block), 0,
-                       s->stream, a, dmag, fmag);
+    // the n-linear model breaks by regimes, not by the sign mode
+    const bool tens = DM != DM_NLIN && a.absmask == 0xffffffffu;
+    if (a.has_corr && tens)
+        hipLaunchKernelGGL((k_step<DM, LPN, true, DM != DM_NLIN>), dim3(grid),
+                           dim3(block), 0, s->stream, a, dmag, fmag);
+    else if (a.has_corr)
+        hipLaunchKernelGGL((k_step<DM, LPN, true, false>), dim3(grid),
+                           dim3(block), 0, s->stream, a, dmag, fmag);
+    else if (tens)
+        hipLaunchKernelGGL((k_step<DM, LPN, false, DM != DM_NLIN>),
+                           dim3(grid), dim3(block), 0, s->stream, a, dmag,
+                           fmag);
+    else
+        hipLaunchKernelGGL((k_step<DM, LPN, false, false>), dim3(grid),
+                           dim3(block), 0, s->stream, a, dmag, fmag);
     return hipGetLastError();
 }
 
