@@ -314,6 +314,7 @@ class Integrator(ABC):
         (the ``tip_types`` of model.py:1093-1212), order kept."""
         self._need_sim()
         self._tip_names = list(tip_types)
+        self._tips = {name: list(tip_types[name]) for name in tip_types}
         node, dof, group = [], [], []
         for g, name in enumerate(self._tip_names):
             for i, j in tip_types[name]:
@@ -488,6 +489,33 @@ class Euler(Integrator):
                                critical_stretch, plus_cs, u, ud, udd, force,
                                body_force, damage, regimes, nregimes,
                                nbond_types)
+        # the Cython Euler keeps the body_force array it is given and never
+        # fills it (integrators.py:371-380, 456-459): write returns it as is
+        self._body_force = (
+            np.zeros((self.nnodes, 3)) if body_force is None else
+            np.array(body_force, dtype=np.float64).reshape(self.nnodes, 3))
+
+    def write(self, u, ud, udd, force, body_force, damage, nlist, n_neigh):
+        """integrators.py:456-459: the state, with body_force as given to
+        create_buffers."""
+        out = super().write(u, ud, udd, force, body_force, damage, nlist,
+                            n_neigh)
+        out[4][...] = self._body_force
+        return out
+
+    def monitor(self):
+        """:meth:`Integrator.monitor` with the body-force sums of the
+        unchanged body_force array (numpy's sum for the model, entry order
+        for the tips, as the reference's write loop computes them)."""
+        model, tips = super().monitor()
+        v = self.volume
+        model["body_force"] = np.sum(self._body_force * v[:, np.newaxis])
+        for name, entries in getattr(self, "_tips", {}).items():
+            acc = 0.0
+            for i, j in entries:
+                acc += self._body_force[i, j] * v[i]
+            tips[name]["body_force"] = acc
+        return model, tips
 
     def _build_special(self):
         """There are no special programs for this integrator."""

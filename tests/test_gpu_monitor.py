@@ -132,3 +132,29 @@ def test_monitor_direct_sums(golden):
     for k, r, sc in zip(integ._SUMS, ref, scale):
         assert abs(sums[k] - r) <= 1e-14 * sc + 1e-300, k
     assert abs(sums["damage_sum"] - np.sum(damage)) <= 1e-12
+
+
+def test_cython_euler_keeps_the_body_force_it_was_given(golden, tmp_path):
+    """integrators.py:371-380, 456-459: the Cython Euler never fills
+    body_force, so its write returns the array it was given (zeros from
+    simulate) and the written body-force sums are 0; ud and udd likewise."""
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        m = Model(golden / "example_mesh.vtk", it.Euler(1e-3),
+                  horizon=pc.HORIZON, critical_stretch=0.005,
+                  bond_stiffness=pc.BOND_STIFFNESS,
+                  is_displacement_boundary=pc.simple_displacement_boundary,
+                  is_force_boundary=is_force_boundary, is_tip=is_tip)
+    (tmp_path / "w").mkdir()
+    out = m.simulate(10, displacement_bc_magnitudes=1e-5 * np.arange(1, 11),
+                     force_bc_magnitudes=np.ones(10), write=5,
+                     write_path=tmp_path / "w")
+    data = out[5]
+    assert np.all(data["model"]["body_force"] == 0.0)
+    for tip in ("right", "edge", "top"):
+        assert np.all(data[tip]["body_force"] == 0.0)
+        assert np.all(data[tip]["velocity"] == 0.0)
+    assert np.any(data["model"]["force"] != 0.0)
+    st = m.integrator.write(None, None, None, None, None, None, None, None)
+    assert np.all(st[4] == 0.0) and np.all(st[1] == 0.0)
+    assert np.any(st[3] != 0.0)
