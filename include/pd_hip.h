@@ -21,6 +21,15 @@
  *   pd_write             <- Integrator.write, integrators.py:285-300
  *   pd_destroy           <- (Python garbage collection of the integrator)
  *
+ * and the set-up the reference runs before the first step:
+ *
+ *   pd_neighbour_list(_device)  <- Model._set_neighbour_list, model.py:536-597
+ *   pd_line_crack_pairs         <- initial_crack_helper, model.py:1578-1612
+ *   pd_create_crack             <- create_crack.pyx:1-43
+ *   pd_set_*_correction,
+ *   pd_set_micromodulus_function <- correction.pyx:5-277
+ *   pd_set_tips, pd_monitor     <- the per-write sums of model.py:1303-1360
+ *
  * Kernel semantics: see the per-function comments and
  * probabilistic_peridynamics_amd/csrc/pd_hip.hip.
  */
