@@ -158,6 +158,7 @@ struct StepArgs {
     double *u, *ud, *udd, *force, *body_force;
     const int *flags;
     const double *bcv, *fbcv, *dens;
+    const double *dens1;  // per-node density when all three DOFs share it
     const int *cnt;
     unsigned short *loc;
     int *gidx;
@@ -529,7 +530,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         pu = a.u[k];
         if (a.integrator != PD_EULER) {
             pud = a.ud[k];
-            pdens = a.dens[k];
+            pdens = a.dens1 ? a.dens1[i] : a.dens[k];
             if (a.integrator == PD_VELOCITY_VERLET)
                 pudd = a.udd[k];
         }
@@ -612,8 +613,12 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         const double bf = d == 0 ? fx : (d == 1 ? fy : fz);
         // force boundary condition after the bond sum
         // (peridynamics.pyx:93-97, cl/peridynamics.cl:117-129)
-        const double F = (fl >> (3 + d)) & 1 ? bf + fmag * a.fbcv[k] : bf;
-        a.body_force[k] = bf;
+        // body_force differs from force only on a force-BC DOF, so only
+        // those store it; reads take force elsewhere
+        const bool fbc = (fl >> (3 + d)) & 1;
+        const double F = fbc ? bf + fmag * a.fbcv[k] : bf;
+        if (fbc)
+            a.body_force[k] = bf;
         a.force[k] = F;
         const bool fixed = (fl >> d) & 1;
         double u = PRE ? pu : a.u[k];
@@ -623,7 +628,8 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         } else if (a.integrator == PD_EULER_CROMER) {
             // cl/euler_cromer.cl:29-34
             double ud = PRE ? pud : a.ud[k];
-            const double dens = PRE ? pdens : a.dens[k];
+            const double dens =
+                PRE ? pdens : (a.dens1 ? a.dens1[i] : a.dens[k]);
             const double udd = (F - a.damping * ud) / dens;
             ud = ud + udd * a.dt;
             a.udd[k] = udd;
@@ -634,7 +640,8 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
             const double hdt = a.dt / 2;
             const double ud1 =
                 (PRE ? pud : a.ud[k]) + hdt * (PRE ? pudd : a.udd[k]);
-            const double dens = PRE ? pdens : a.dens[k];
+            const double dens =
+                PRE ? pdens : (a.dens1 ? a.dens1[i] : a.dens[k]);
             const double udd1 = (F - a.damping * ud1) / dens;
             const double ud = ud1 + hdt * udd1;
             a.ud[k] = ud;
@@ -764,7 +771,7 @@ constexpr int MON_SUMS = 6;
 __global__ void __launch_bounds__(256)
     k_monitor_part(int n, const double *u, const double *ud,
                    const double *udd, const double *force,
-                   const double *body, const double *vol,
+                   const double *body, const int *flags, const double *vol,
                    const double *damage, double *part)
 {
     __shared__ double sh[MON_SUMS][256];
@@ -773,12 +780,14 @@ __global__ void __launch_bounds__(256)
     const size_t step = (size_t)MON_BLOCKS * 256;
     double a[MON_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (size_t q = first; q < 3 * (size_t)n; q += step) {
-        const double v = vol[q / 3];
+        const size_t p = q / 3;
+        const double v = vol[p];
+        const bool fbc = (flags[p] >> (3 + (int)(q - 3 * p))) & 1;
         a[0] += u[q];
         a[1] += ud[q];
         a[2] += udd[q];
         a[3] += force[q] * v;
-        a[4] += body[q] * v;
+        a[4] += (fbc ? body[q] : force[q]) * v;
     }
     for (size_t i = first; i < (size_t)n; i += step)
         a[5] += damage[i];
@@ -803,7 +812,8 @@ __global__ void __launch_bounds__(256)
     k_monitor_final(const double *part, int ngroups, const int *tip_off,
                     const int *tip_q, const double *u, const double *ud,
                     const double *udd, const double *force,
-                    const double *body, const double *vol, double *out)
+                    const double *body, const int *flags, const double *vol,
+                    double *out)
 {
     const int t = threadIdx.x;
     if (t < MON_SUMS) {
@@ -816,12 +826,14 @@ __global__ void __launch_bounds__(256)
         double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int e = tip_off[g]; e < tip_off[g + 1]; ++e) {
             const int q = tip_q[e];
-            const double v = vol[q / 3];
+            const int p = q / 3;
+            const double v = vol[p];
+            const bool fbc = (flags[p] >> (3 + q - 3 * p)) & 1;
             a[0] += u[q];
             a[1] += ud[q];
             a[2] += udd[q];
             a[3] += force[q] * v;
-            a[4] += body[q] * v;
+            a[4] += (fbc ? body[q] : force[q]) * v;
         }
         for (int k = 0; k < 5; ++k)
             out[MON_SUMS + 5 * g + k] = a[k];
@@ -1069,6 +1081,7 @@ struct pd_sim {
 
     DevBuf<NodeRec> rec[2];
     DevBuf<TileInfo> tiles;
+    DevBuf<double> dens1;
     DevBuf<double> r0, vol, u, ud, udd, force, body_force, bcv, fbcv, dens,
         damage, cnode, corr, k_tab, s_tab, pc_tab;
     DevBuf<int> flags, family, cnt, tnode, halo, gidx, stride, nlist_ref,
@@ -1204,6 +1217,7 @@ StepArgs make_args(pd_sim *s)
     a.bcv = s->bcv.p;
     a.fbcv = s->fbcv.p;
     a.dens = s->dens.p;
+    a.dens1 = s->dens1.p;
     a.cnt = s->cnt.p;
     a.loc = s->loc.p;
     a.gidx = s->gidx.p;
@@ -1854,8 +1868,22 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     chk(upload_perm(s->bcv, s->perm, s->h_bcv.data(), 3), "bc_values");
     chk(upload_perm(s->fbcv, s->perm, s->h_fbcv.data(), 3),
         "force_bc_values");
-    if (!s->h_dens.empty())
+    s->dens1.release();
+    if (!s->h_dens.empty()) {
         chk(upload_perm(s->dens, s->perm, s->h_dens.data(), 3), "densities");
+        // one density per node (the reference's tiled layout, model.py:
+        // 691-702) is read as one value per node
+        bool per_node = true;
+        for (size_t i = 0; per_node && i < N; ++i)
+            per_node = s->h_dens[3 * i] == s->h_dens[3 * i + 1] &&
+                       s->h_dens[3 * i] == s->h_dens[3 * i + 2];
+        if (per_node) {
+            std::vector<double> d1(N);
+            for (size_t i = 0; i < N; ++i)
+                d1[i] = s->h_dens[3 * i];
+            chk(upload_perm(s->dens1, s->perm, d1.data(), 1), "densities");
+        }
+    }
     if (s->has_cnode)
         chk(upload_perm(s->cnode, s->perm, s->h_cnode.data(), 1),
             "node stiffness");
@@ -1988,13 +2016,13 @@ int pd_monitor(pd_sim *s, double *model, double *tips)
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_monitor_part, dim3(MON_BLOCKS), dim3(256), 0,
                        s->stream, N, s->u.p, s->ud.p, s->udd.p, s->force.p,
-                       s->body_force.p, s->vol.p, s->damage.p,
+                       s->body_force.p, s->flags.p, s->vol.p, s->damage.p,
                        s->mon_part.p);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s->stream,
                        s->mon_part.p, ngroups, s->tip_off.p, s->tip_q.p,
                        s->u.p, s->ud.p, s->udd.p, s->force.p,
-                       s->body_force.p, s->vol.p, s->mon_out.p);
+                       s->body_force.p, s->flags.p, s->vol.p, s->mon_out.p);
     HIPCHK(hipGetLastError());
     std::vector<double> h(nout);
     HIPCHK(hipMemcpyAsync(h.data(), s->mon_out.p, nout * sizeof(double),
@@ -2370,8 +2398,20 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
         HIPCHK(download_perm(udd, s->udd, s->perm, 3));
     if (force)
         HIPCHK(download_perm(force, s->force, s->perm, 3));
-    if (body_force)
+    if (body_force) {
+        // stored only on force-BC DOFs; equal to force everywhere else
         HIPCHK(download_perm(body_force, s->body_force, s->perm, 3));
+        std::vector<double> tmp;
+        const double *f = force;
+        if (!f) {
+            tmp.resize(3 * (size_t)N);
+            HIPCHK(download_perm(tmp.data(), s->force, s->perm, 3));
+            f = tmp.data();
+        }
+        for (size_t q = 0; q < 3 * (size_t)N; ++q)
+            if (!((s->h_flags[q / 3] >> (3 + (int)(q % 3))) & 1))
+                body_force[q] = f[q];
+    }
     if (damage)
         HIPCHK(hipMemcpy(damage, s->damage.p, (size_t)N * sizeof(double),
                          hipMemcpyDeviceToHost));
