@@ -357,9 +357,13 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
             const int base = (st[q] >> 8) * a.nregimes;
             const int reg = st[q] & 0xff;
             const double c_hi = ts[base + reg];
-            const double c_lo = reg > 0 ? ts[base + reg - 1] : -__builtin_inf();
-            near = fabs(sh[q] - c_hi) <= GUARD * fabs(c_hi) ||
-                   fabs(sh[q] - c_lo) <= GUARD * fabs(c_lo);
+            // regime 0 has no lower threshold (an infinite one would make
+            // every regime-0 bond "near" it: inf <= GUARD * inf)
+            near = fabs(sh[q] - c_hi) <= GUARD * fabs(c_hi);
+            if (reg > 0) {
+                const double c_lo = ts[base + reg - 1];
+                near |= fabs(sh[q] - c_lo) <= GUARD * fabs(c_lo);
+            }
         }
         slow[q] = live[q] & (!fast | near);
     }
