@@ -125,8 +125,8 @@ int pd_set_break_mode(pd_sim *sim, int tension_only);
  * Defaults: 4 and 128.  lanes_per_node takes effect at the next step;
  * tile_nodes at the next pd_create_buffers, which builds the tiles (stepping
  * in between fails with PD_E_STATE).  A tile's
- * halo (its nodes and all their neighbours) is capped at 1280 entries (1184
- * for n-linear models), so a node may have at most 1279 (1183) bonds. */
+ * halo (its nodes and all their neighbours) is capped at 1280 entries (1152
+ * for n-linear models), so a node may have at most 1279 (1151) bonds. */
 int pd_set_launch(pd_sim *sim, int lanes_per_node, int tile_nodes);
 
 /* Tiling of the current state: number of tiles, halo capacity per tile

@@ -126,7 +126,7 @@ constexpr int XCDS = 8;
 // LDS banks.  4*16*cap bytes (plus the n-linear tables) stay within 80 KiB,
 // which lets two workgroups share a CU's 160 KiB.
 constexpr int LDS_CAP_PMB = 1280;
-constexpr int LDS_CAP_NLIN = 1184;
+constexpr int LDS_CAP_NLIN = 1152;
 constexpr unsigned LDS_BYTES = 81920;
 
 // damage models of the step kernel
@@ -354,16 +354,12 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
         if (DM != DM_NLIN) {
             near = fabs((TENS ? sh[q] : fabs(sh[q])) - a.s0h) <= a.s0h_guard;
         } else {
-            const int base = (st[q] >> 8) * a.nregimes;
-            const int reg = st[q] & 0xff;
-            const double c_hi = ts[base + reg];
-            // regime 0 has no lower threshold (an infinite one would make
-            // every regime-0 bond "near" it: inf <= GUARD * inf)
-            near = fabs(sh[q] - c_hi) <= GUARD * fabs(c_hi);
-            if (reg > 0) {
-                const double c_lo = ts[base + reg - 1];
-                near |= fabs(sh[q] - c_lo) <= GUARD * fabs(c_lo);
-            }
+            // (lower, upper) thresholds of the bond's regime; regime 0's
+            // lower one is -DBL_MAX, which no stretch is near
+            const double2 c = reinterpret_cast<const double2 *>(
+                ts)[(st[q] >> 8) * a.nregimes + (st[q] & 0xff)];
+            near = fabs(sh[q] - c.y) <= GUARD * fabs(c.y) ||
+                   fabs(sh[q] - c.x) <= GUARD * fabs(c.x);
         }
         slow[q] = live[q] & (!fast | near);
     }
@@ -412,8 +408,9 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
             const int t = st[q] >> 8;
             const int reg = st[q] & 0xff;
             const int base = t * a.nregimes;
-            const double c_hi = ts[base + reg];
-            const double c_lo = reg > 0 ? ts[base + reg - 1] : -__builtin_inf();
+            const double2 c =
+                reinterpret_cast<const double2 *>(ts)[base + reg];
+            const double c_hi = c.y, c_lo = c.x;
             int nreg = reg;
             if (sh[q] < c_hi) {
                 if (reg > 0 && sh[q] < c_lo)
@@ -431,8 +428,10 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
                 ++nbroken;
                 continue;
             }
-            const double fn =
-                __builtin_fma(sh[q] + sh[q], tk[base + nreg], tp[base + nreg]);
+            // (bond stiffness, plus_cs) of the new regime
+            const double2 kp =
+                reinterpret_cast<const double2 *>(tk)[base + nreg];
+            const double fn = __builtin_fma(sh[q] + sh[q], kp.x, kp.y);
             g = fn * (hl[q] + hl[q]) * w[q];
         }
         if (CORR)
@@ -567,13 +566,19 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     if (threadIdx.x < 4)  // entry 0: the zero record dead slots read
         *reinterpret_cast<double2 *>(ldsb + threadIdx.x * PB) =
             make_double2(0.0, 0.0);
-    double *ts = reinterpret_cast<double *>(ldsb + 4 * PB);
-    double *tk = ts + MAX_TAB, *tp = tk + MAX_TAB;
+    // n-linear tables, one 16-byte read each: (lower, upper) halved
+    // thresholds of regime r of type t, and (stiffness, plus_cs)
+    double2 *tsp = reinterpret_cast<double2 *>(ldsb + 4 * PB);
+    double2 *tkp = tsp + MAX_TAB;
+    const double *ts = reinterpret_cast<const double *>(tsp);
+    const double *tk = reinterpret_cast<const double *>(tkp);
+    const double *tp = nullptr;
     if (DM == DM_NLIN) {
         for (int t = threadIdx.x; t < a.ntab; t += blockDim.x) {
-            tk[t] = a.k_tab[t];
-            ts[t] = a.s_tab[t] * 0.5;
-            tp[t] = a.pc_tab[t];
+            const bool first = t % a.nregimes == 0;
+            tsp[t] = make_double2(first ? -__DBL_MAX__ : a.s_tab[t - 1] * 0.5,
+                                  a.s_tab[t] * 0.5);
+            tkp[t] = make_double2(a.k_tab[t], a.pc_tab[t]);
         }
     }
     __syncthreads();
