@@ -202,17 +202,6 @@ __device__ __forceinline__ void sqrt_core(double q, double &g, double &h)
     g = __builtin_fma(d, h, g);
 }
 
-// 1/sqrt(q) for q in [2^-767, 2^767]: the hardware estimate and two Newton
-// steps y <- y (3/2 - q y^2 / 2).
-__device__ __forceinline__ double rsqrt_newton(double q)
-{
-    double y = __builtin_amdgcn_rsq(q);
-    const double hq = 0.5 * q;
-    y = y * __builtin_fma(-hq * y, y, 1.5);
-    y = y * __builtin_fma(-hq * y, y, 1.5);
-    return y;
-}
-
 __device__ __forceinline__ bool core_range(double q)
 {
     return (q >= 0x1p-767) & (q < __builtin_inf());
@@ -960,6 +949,17 @@ __global__ void k_adj_jint(int n, const int *cnt, const long long *chunk0,
         const size_t s = slot_of(chunk0[p], stride[p], k);
         jint[s] = inv[(uint32_t)gidx[s] & KEYM];
     }
+}
+
+// 1/sqrt(q) for q in [2^-767, 2^767]: the hardware estimate and two Newton
+// steps y <- y (3/2 - q y^2 / 2).
+__device__ __forceinline__ double rsqrt_newton(double q)
+{
+    double y = __builtin_amdgcn_rsq(q);
+    const double hq = 0.5 * q;
+    y = y * __builtin_fma(-hq * y, y, 1.5);
+    y = y * __builtin_fma(-hq * y, y, 1.5);
+    return y;
 }
 
 // One backward step: ADJ_LPN lanes per node (bonds round-robin, xor-shuffle
