@@ -464,3 +464,78 @@ def test_full_size_momentum_and_restart(name):
     other = configs.make_integrator(w, lanes_per_node=2, tile_nodes=256)
     other.run(12, d, f)
     assert pc.rel_l2(_state(other)["u"], end["u"]) < 1e-12
+
+
+def _bar_sensitivity(w, steps=20):
+    """Relative L2 change of the oracle's own u after ``steps`` steps when
+    every free displacement is moved by one ulp, up or down at random,
+    after step 2: how far the run's own rounding determines u."""
+    def run(perturb):
+        ref = oracle.SlotStepper(0, w.dt, w.coords, w.volume, w.family,
+                                 w.bc_types, w.bc_values, w.force_bc_types,
+                                 w.force_bc_values, w.nlist,
+                                 w.bond_stiffness, w.critical_stretch)
+        d, f = w.magnitudes(steps)
+        for s in range(steps):
+            ref(d[s], f[s])
+            if s == 1 and perturb:
+                up = np.random.default_rng(0).random(ref.u.shape) < 0.5
+                free = w.bc_types == 0
+                ref.u[free & up] = np.nextafter(ref.u[free & up], np.inf)
+                ref.u[free & ~up] = np.nextafter(ref.u[free & ~up],
+                                                 -np.inf)
+        return ref.u
+    return pc.rel_l2(run(True), run(False))
+
+
+@pytest.mark.parametrize("layout", ["compact", "slot"])
+def test_bar_workload_full_length(layout):
+    """The 2560-node pre-cracked bar exactly as specified (Euler, dt = 1e-5,
+    1000 steps, pull 1e-8 per step).  The step is far past the stability
+    limit: the pull throws the free end metres in two steps and a cascade
+    breaks 27,328 bonds in step 3, after which the state stays put.  Lists,
+    counts and damage equal the oracle's exactly.  u cannot be held to
+    1e-10 here: bonds between nodes thrown metres get their stretch from
+    differences of large positions, so one ulp of u after step 2 (moved at
+    random) changes the oracle's own final u by ~7e-10.  u is held to 50
+    times that sensitivity, measured in the test."""
+    w = configs.bar_euler(layout=layout)
+    if layout == "compact":
+        integ = it.Euler(w.dt)
+        integ.build(w.nnodes, 3, w.max_neighbours, w.coords, w.volume,
+                    w.family, w.bc_types, w.bc_values, w.force_bc_types,
+                    w.force_bc_values, None, None, None)
+        z = np.zeros((w.nnodes, 3))
+        integ.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,
+                             w.critical_stretch, None, z, z, z, z, z,
+                             np.zeros(w.nnodes), None, 1, 1)
+        ref = oracle.CythonEuler(w.dt, w.coords, w.volume, w.family,
+                                 w.bc_types, w.bc_values, w.force_bc_types,
+                                 w.force_bc_values, w.nlist, w.n_neigh,
+                                 w.bond_stiffness, w.critical_stretch)
+    else:
+        integ = configs.make_integrator(w)
+        ref = oracle.SlotStepper(0, w.dt, w.coords, w.volume, w.family,
+                                 w.bc_types, w.bc_values, w.force_bc_types,
+                                 w.force_bc_values, w.nlist,
+                                 w.bond_stiffness, w.critical_stretch)
+    d, f = w.magnitudes(1000)
+    for s in range(1000):
+        integ(d[s], f[s])
+        ref(d[s], f[s])
+        if s < 4 or s == 999:
+            got = _state(integ)
+            if layout == "compact":
+                rn, rd = ref.n_neigh, ref.damage()
+            else:
+                rn, rd = ref.connectivity()
+            np.testing.assert_array_equal(got["nlist"], ref.nlist)
+            np.testing.assert_array_equal(got["n_neigh"], rn)
+            np.testing.assert_array_equal(got["damage"], rd)
+    assert int(w.n_neigh.sum() - rn.sum()) == 27328
+    sens = _bar_sensitivity(configs.bar_euler(layout="slot"))
+    err = pc.rel_l2(got["u"], ref.u)
+    print(f"bar, {layout}: u rel-L2 {err:.2e}, one-ulp sensitivity "
+          f"{sens:.2e}")
+    assert sens > 1e-10  # the run is this ill-conditioned
+    assert err < 50 * sens
