@@ -765,9 +765,25 @@ class Model(object):
         monitor = getattr(self.integrator, "monitor", None)
         if write and monitor is not None:
             self.integrator.set_tips(self.tip_types)
-        for step in range(first_step, first_step + steps):
-            self.integrator(displacement_bc_magnitudes[step - 1],
-                            force_bc_magnitudes[step - 1])
+        # the steps between two writes go to the library's own step loop in
+        # one call (the same launches as one call per step) unless the
+        # integrator's step is overridden in Python
+        run = getattr(self.integrator, "run", None)
+        if (run is not None and type(self.integrator).__call__.__module__
+                != Integrator.__module__):
+            run = None
+        last = first_step + steps - 1
+        step = first_step
+        while step <= last:
+            if run is not None:
+                end = last if not write else min(last,
+                                                 -(-step // write) * write)
+                run(end - step + 1, displacement_bc_magnitudes[step - 1:end],
+                    force_bc_magnitudes[step - 1:end])
+                step = end
+            else:
+                self.integrator(displacement_bc_magnitudes[step - 1],
+                                force_bc_magnitudes[step - 1])
             if write and step % write == 0 and monitor is not None:
                 u, damage = self.integrator.write_fields(u, damage)
                 self.write_mesh(write_path / f"U_{step}.vtk", damage, u)
@@ -821,6 +837,7 @@ class Model(object):
                 if damage_sum > 0.05 * self.nnodes:
                     warnings.warn("Over 5% of bonds have broken! "
                                   "peridynamics simulation continuing")
+            step += 1
         for tip_type_str in data:
             ntip = self.ntips[tip_type_str]
             if ntip != 0:

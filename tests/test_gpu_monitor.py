@@ -158,3 +158,29 @@ def test_cython_euler_keeps_the_body_force_it_was_given(golden, tmp_path):
     st = m.integrator.write(None, None, None, None, None, None, None, None)
     assert np.all(st[4] == 0.0) and np.all(st[1] == 0.0)
     assert np.any(st[3] != 0.0)
+
+
+def test_simulate_in_library_loops_equals_per_step_calls(golden, tmp_path):
+    """Model.simulate hands the steps between two writes to the library's
+    step loop; the result equals one Python call per step bitwise,
+    whatever the write period, first step and run length."""
+    mags = 1e-5 * np.arange(1, 40)
+    outs = []
+    for per_step in (False, True):
+        m = _model(golden)
+        if per_step:
+            m.integrator.run = None
+        path = tmp_path / ("step" if per_step else "loop")
+        path.mkdir()
+        out = m.simulate(23, displacement_bc_magnitudes=mags,
+                         force_bc_magnitudes=np.ones(39), write=7,
+                         first_step=4, write_path=path)
+        outs.append((out, sorted(p.name for p in path.iterdir())))
+    (a, fa), (b, fb) = outs
+    assert fa == fb == ["U_14.vtk", "U_21.vtk", "U_7.vtk"]
+    for x, y in zip(a[:2] + a[3:5], b[:2] + b[3:5]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a[2][0], b[2][0])
+    for key in a[5]:
+        for k, v in a[5][key].items():
+            assert np.array_equal(v, b[5][key][k]), (key, k)
