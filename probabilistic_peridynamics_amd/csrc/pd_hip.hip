@@ -936,21 +936,6 @@ __global__ void k_adj_init(int n, const int *cnt, const long long *chunk0,
     }
 }
 
-// Internal index of each slot's neighbour (inv: caller id -> internal id).
-__global__ void k_adj_jint(int n, const int *cnt, const long long *chunk0,
-                           const int *stride, const int *gidx,
-                           const int *inv, int *jint)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n)
-        return;
-    const int c = cnt[p];
-    for (int k = 0; k < c; ++k) {
-        const size_t s = slot_of(chunk0[p], stride[p], k);
-        jint[s] = inv[(uint32_t)gidx[s] & KEYM];
-    }
-}
-
 // 1/sqrt(q) for q in [2^-767, 2^767]: the hardware estimate and two Newton
 // steps y <- y (3/2 - q y^2 / 2).
 __device__ __forceinline__ double rsqrt_newton(double q)
@@ -962,78 +947,170 @@ __device__ __forceinline__ double rsqrt_newton(double q)
     return y;
 }
 
-// One backward step: ADJ_LPN lanes per node (bonds round-robin, xor-shuffle
-// reduction).  u: u_n; lam: lambda_{n+1}; writes lambda_n and adds this
-// step's term of dL/dc.  All arrays in internal node order.
-constexpr int ADJ_LPN = 4;
-__global__ __launch_bounds__(256) void k_adjoint(
-    int n, int step, double dt, const int *cnt, const long long *chunk0,
-    const int *stride, const int *brk, const int *jint, const int *flags,
-    const double *r0, const double *u, const double *vol, const double *c,
-    const double *lam, double *lam_out, double *gradc)
+// 1/l0 of every slot's bond, once per recording (inv: caller id -> internal
+// id); the slots past a node's count keep the 0 they were cleared to.  The
+// same reciprocal square root of the same squared length as the backward
+// kernel's 1/l, so that a bond with l == l0 (bitwise, e.g. both ends at
+// rest) gets 1/l0 - 1/l = 0 exactly, as its stretch is 0 in the forward
+// step: a one-ulp difference there, times k dt, is not small.
+__global__ void k_adj_il0(int n, const int *cnt, const long long *chunk0,
+                          const int *stride, const int *gidx, const int *inv,
+                          const double *r0, double *il0)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = t / ADJ_LPN, sub = t % ADJ_LPN;
-    const bool valid = m < n;
-    const int mm = valid ? m : 0;
-    const size_t b = 3 * (size_t)mm;
-    const int fm = flags[mm];
-    double mu_m[3], rm[3], r0m[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        mu_m[d] = (fm >> d) & 1 ? 0.0 : lam[b + d];
-        r0m[d] = r0[b + d];
-        rm[d] = r0m[d] + u[b + d];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int c = cnt[p];
+    const size_t b = 3 * (size_t)p;
+    for (int k = 0; k < c; ++k) {
+        const size_t s = slot_of(chunk0[p], stride[p], k);
+        const size_t bj = 3 * (size_t)inv[(uint32_t)gidx[s] & KEYM];
+        const double ex = r0[bj] - r0[b];
+        const double ey = r0[bj + 1] - r0[b + 1];
+        const double ez = r0[bj + 2] - r0[b + 2];
+        il0[s] = rsqrt_newton((ex * ex + ey * ey) + ez * ez);
     }
-    const double vm = vol[mm], cm = c[mm];
+}
+
+// The per-node inputs of one backward step as 64-byte records, from which
+// the tiles stage their halos: a = (rx, ry, rz, V), b = (mux, muy, muz, c)
+// with r = r0 + u_n and mu = lambda_{n+1} with prescribed DOFs zeroed.
+__global__ void k_adj_pack(int n, const double *r0, const double *u,
+                           const double *vol, const double *c,
+                           const int *flags, const double *lam, NodeRec *rec)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const size_t b = 3 * (size_t)p;
+    const int f = flags[p];
+    NodeRec o;
+    o.a = make_double4(r0[b] + u[b], r0[b + 1] + u[b + 1],
+                       r0[b + 2] + u[b + 2], vol[p]);
+    o.b = make_double4(f & 1 ? 0.0 : lam[b], f & 2 ? 0.0 : lam[b + 1],
+                       f & 4 ? 0.0 : lam[b + 2], c[p]);
+    rec[p] = o;
+}
+
+struct AdjArgs {
+    int ntiles, tiles_per_xcd, step;
+    double dt;
+    const TileInfo *tiles;
+    const int *halo, *cnt, *brk;
+    const NodeRec *rec;  // k_adj_pack's records of this step
+    const unsigned short *loc0;  // loc at the start of the recording
+    const double *il0;
+    double *lam_out, *gradc;
+};
+
+// One backward step on the step kernel's tiles: the tile's halo records
+// (k_adj_pack) are staged into LDS piece-major, as in k_step, and every bond
+// of a node reads its neighbour from there.  LPN lanes per node take its
+// 4-bond chunks round-robin; a slot is live in step n if it held a bond at
+// the start of the recording (loc0) that broke at a step > n (brk).
+template <int LPN>
+__global__ __launch_bounds__(1024) void k_adj_tile(AdjArgs a)
+{
+    __shared__ double2 lds[LDS_BYTES / sizeof(double2)];
+    const int tile = (blockIdx.x % XCDS) * a.tiles_per_xcd + blockIdx.x / XCDS;
+    if (tile >= a.ntiles)
+        return;
+    const TileInfo ti = a.tiles[tile];
+    char *ldsb = reinterpret_cast<char *>(lds);
+    const int l = threadIdx.x / LPN;
+    const int sub = threadIdx.x % LPN;
+    const bool valid = l < ti.nodes;
+    const uint32_t i = (uint32_t)(ti.node_off + (valid ? l : 0));
+    const NodeRec own = a.rec[i];
+    const int cnt = valid ? a.cnt[i] : 0;
+
+    constexpr int SP = 10;
+    constexpr unsigned PB = 16u * LDS_CAP_PMB;
+    const int np = 4 * ti.halo;
+    for (int x0 = threadIdx.x; x0 < np; x0 += SP * blockDim.x) {
+        double2 v[SP];
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            const int x = min(x0 + k * (int)blockDim.x, np - 1);
+            const int g = a.halo[ti.halo_off + (x >> 2)];
+            v[k] = reinterpret_cast<const double2 *>(a.rec + g)[x & 3];
+        }
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            const int x = x0 + k * (int)blockDim.x;
+            if (x < np)
+                *reinterpret_cast<double2 *>(
+                    ldsb + (x & 3) * PB + 16u * (unsigned)((x >> 2) + 1)) =
+                    v[k];
+        }
+    }
+    __syncthreads();
+
+    const double rm[3] = {own.a.x, own.a.y, own.a.z};
+    const double mum[3] = {own.b.x, own.b.y, own.b.z};
+    const double vm = own.a.w, cm = own.b.w;
     double jt[3] = {0.0, 0.0, 0.0};
     double gc = 0.0;
-    const int cn = valid ? cnt[mm] : 0;
-    for (int k = sub; k < cn; k += ADJ_LPN) {
-        const size_t s = slot_of(chunk0[mm], stride[mm], k);
-        if (brk[s] <= step)
-            continue;  // broken before or in this step: no force in step n
-        const int j = jint[s];
-        const size_t bj = 3 * (size_t)j;
-        const int fj = flags[j];
-        const double vj = vol[j];
-        double dd[3], d0[3], w[3];
+    const ushort4 *L4 = reinterpret_cast<const ushort4 *>(a.loc0);
+    const int4 *B4 = reinterpret_cast<const int4 *>(a.brk);
+    const double2 *I2 = reinterpret_cast<const double2 *>(a.il0);
+    const long long c0 = ti.chunk_off + l;
+    const int nch = (cnt + 3) >> 2;
+    for (int c = sub; c < nch; c += LPN) {
+        const long long k = c0 + (long long)c * ti.nodes;
+        const ushort4 lc = L4[k];
+        const int4 bk = B4[k];
+        const double2 ia = I2[2 * k], ib = I2[2 * k + 1];
+        const unsigned ev[4] = {lc.x, lc.y, lc.z, lc.w};
+        const int br[4] = {bk.x, bk.y, bk.z, bk.w};
+        const double il0[4] = {ia.x, ia.y, ib.x, ib.y};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            dd[d] = (r0[bj + d] + u[bj + d]) - rm[d];
-            d0[d] = r0[bj + d] - r0m[d];
-            const double muj = (fj >> d) & 1 ? 0.0 : lam[bj + d];
-            w[d] = vm * muj - vj * mu_m[d];
+        for (int q = 0; q < 4; ++q) {
+            if (ev[q] == LOC_DEAD || br[q] <= a.step)
+                continue;  // empty, or broken before or in this step
+            const char *pe = ldsb + 16u * ev[q];
+            const double2 p0 = *reinterpret_cast<const double2 *>(pe);
+            const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
+            const double2 p2 =
+                *reinterpret_cast<const double2 *>(pe + 2 * PB);
+            const double2 p3 =
+                *reinterpret_cast<const double2 *>(pe + 3 * PB);
+            const double dd[3] = {p0.x - rm[0], p0.y - rm[1], p1.x - rm[2]};
+            const double muj[3] = {p2.x, p2.y, p3.x};
+            const double vj = p1.y, cj = p3.y;
+            double w[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                w[d] = vm * muj[d] - vj * mum[d];
+            // 1/l from the reciprocal square root of the squared length,
+            // two Newton steps (relative error ~1e-16): no square root or
+            // division in the loop
+            const double il =
+                rsqrt_newton((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+            const double psi = il0[q] - il;
+            const double kk = (cm + cj) * 0.5;
+            const double dw = dd[0] * w[0] + dd[1] * w[1] + dd[2] * w[2];
+            const double il3 = il * il * il;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                jt[d] += kk * (psi * w[d] + dd[d] * dw * il3);
+            gc -= 0.5 * psi * dw;
         }
-        // 1/l and 1/l0 from the reciprocal square root of the squared
-        // lengths, two Newton steps (relative error ~1e-16): no square
-        // root or division is needed
-        const double il =
-            rsqrt_newton((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
-        const double il0 =
-            rsqrt_newton((d0[0] * d0[0] + d0[1] * d0[1]) + d0[2] * d0[2]);
-        const double psi = il0 - il;
-        const double kk = (cm + c[j]) * 0.5;
-        const double dw = dd[0] * w[0] + dd[1] * w[1] + dd[2] * w[2];
-        const double il3 = il * il * il;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            jt[d] += kk * (psi * w[d] + dd[d] * dw * il3);
-        gc -= 0.5 * psi * dw;
     }
 #pragma unroll
-    for (int off = ADJ_LPN / 2; off > 0; off >>= 1) {
-        jt[0] += __shfl_xor(jt[0], off, ADJ_LPN);
-        jt[1] += __shfl_xor(jt[1], off, ADJ_LPN);
-        jt[2] += __shfl_xor(jt[2], off, ADJ_LPN);
-        gc += __shfl_xor(gc, off, ADJ_LPN);
+    for (int off = LPN / 2; off > 0; off >>= 1) {
+        jt[0] += __shfl_xor(jt[0], off, LPN);
+        jt[1] += __shfl_xor(jt[1], off, LPN);
+        jt[2] += __shfl_xor(jt[2], off, LPN);
+        gc += __shfl_xor(gc, off, LPN);
     }
     if (!valid)
         return;
-    for (int d = sub; d < 3; d += ADJ_LPN)
-        lam_out[b + d] = mu_m[d] + dt * jt[d];
+    const size_t b = 3 * (size_t)i;
+    for (int d = sub; d < 3; d += LPN)
+        a.lam_out[b + d] = mum[d] + a.dt * jt[d];
     if (sub == 0)
-        gradc[mm] += dt * gc;
+        a.gradc[i] += a.dt * gc;
 }
 
 }  // namespace pdk
@@ -1117,8 +1194,10 @@ struct pd_sim {
     bool adj_recording = false, adj_ready = false, adj_replay = false;
     int adj_nsteps = 0, adj_every = 0, step_counter = 0;
     std::vector<double> adj_dmag, adj_fmag;
-    DevBuf<int> brk, jint;
+    DevBuf<int> brk;
     DevBuf<unsigned short> loc0;
+    DevBuf<double> il0;    // 1/l0 per slot
+    DevBuf<NodeRec> arec;  // per-step records of the backward kernel
     DevBuf<double> ckpt, seg, lam[2], gradc, fin_u, fin_force, fin_bforce;
 
     // monitored sums: tip entries (caller's node, dof) grouped, in order
@@ -1298,6 +1377,31 @@ hipError_t launch_dm(pd_sim *s, const StepArgs &a, double dmag, double fmag)
 }
 
 
+// One backward step's k_adj_tile, with the step kernel's grid and block.
+hipError_t launch_adjoint(pd_sim *s, const AdjArgs &a)
+{
+    const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
+    switch (s->lpn) {
+    case 1:
+        hipLaunchKernelGGL(k_adj_tile<1>, dim3(grid), dim3(s->tile_nodes), 0,
+                           s->stream, a);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_adj_tile<2>, dim3(grid),
+                           dim3(2 * s->tile_nodes), 0, s->stream, a);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_adj_tile<4>, dim3(grid),
+                           dim3(4 * s->tile_nodes), 0, s->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_adj_tile<8>, dim3(grid),
+                           dim3(8 * s->tile_nodes), 0, s->stream, a);
+        break;
+    }
+    return hipGetLastError();
+}
+
 void adjoint_release(pd_sim *s)
 {
     s->adj_recording = s->adj_ready = s->adj_replay = false;
@@ -1305,8 +1409,9 @@ void adjoint_release(pd_sim *s)
     s->adj_dmag.clear();
     s->adj_fmag.clear();
     s->brk.release();
-    s->jint.release();
     s->loc0.release();
+    s->il0.release();
+    s->arec.release();
     s->ckpt.release();
     s->seg.release();
     s->lam[0].release();
@@ -2254,7 +2359,8 @@ int pd_adjoint_start(pd_sim *s, int nsteps, int checkpoint_every)
     DevBuf<int> inv_d;
     HIPCHK(upload(inv_d, inv.data(), N));
     HIPCHK(s->brk.alloc(S));
-    HIPCHK(s->jint.alloc(S));
+    HIPCHK(s->il0.alloc(S));
+    HIPCHK(s->arec.alloc(N));
     HIPCHK(s->loc0.alloc(S));
     HIPCHK(s->ckpt.alloc((size_t)nck * B));
     HIPCHK(s->seg.alloc((size_t)checkpoint_every * B));
@@ -2266,14 +2372,17 @@ int pd_adjoint_start(pd_sim *s, int nsteps, int checkpoint_every)
     HIPCHK(s->fin_bforce.alloc(B));
     HIPCHK(hipMemcpyAsync(s->loc0.p, s->loc.p, S * sizeof(unsigned short),
                           hipMemcpyDeviceToDevice, s->stream));
+    // slots past a node's count: brk -1 (all bytes set), 1/l0 zero
+    HIPCHK(hipMemsetAsync(s->brk.p, 0xff, S * sizeof(int), s->stream));
+    HIPCHK(hipMemsetAsync(s->il0.p, 0, S * sizeof(double), s->stream));
     const int blocks = (s->n + 255) / 256;
     hipLaunchKernelGGL(k_adj_init, dim3(blocks), dim3(256), 0, s->stream,
                        s->n, s->cnt.p, s->chunk0.p, s->stride.p, s->loc.p,
                        s->brk.p);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_adj_jint, dim3(blocks), dim3(256), 0, s->stream,
+    hipLaunchKernelGGL(k_adj_il0, dim3(blocks), dim3(256), 0, s->stream,
                        s->n, s->cnt.p, s->chunk0.p, s->stride.p, s->gidx.p,
-                       inv_d.p, s->jint.p);
+                       inv_d.p, s->r0.p, s->il0.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->ckpt.p, s->u.p, B * sizeof(double),
                           hipMemcpyDeviceToDevice, s->stream));
@@ -2330,8 +2439,19 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
         HIPCHK(hipStreamSynchronize(st));
     }
     int cur = 0;
-    const int ablocks = (int)((N * ADJ_LPN + 255) / 256);
-    const double *cfield = s->cnode.p;
+    const int pblocks = (int)((N + 255) / 256);
+    AdjArgs aa{};
+    aa.ntiles = s->ntiles;
+    aa.tiles_per_xcd = (s->ntiles + XCDS - 1) / XCDS;
+    aa.dt = s->dt;
+    aa.tiles = s->tiles.p;
+    aa.halo = s->halo.p;
+    aa.cnt = s->cnt.p;
+    aa.brk = s->brk.p;
+    aa.rec = s->arec.p;
+    aa.loc0 = s->loc0.p;
+    aa.il0 = s->il0.p;
+    aa.gradc = s->gradc.p;
     struct ReplayGuard {  // steps below are replays, also on early returns
         pd_sim *s;
         ~ReplayGuard() { s->adj_replay = false; }
@@ -2357,13 +2477,15 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
                                       st));
         }
         for (int n = n1 - 1; n >= n0 && rc == PD_OK; --n) {
-            hipLaunchKernelGGL(k_adjoint, dim3(ablocks), dim3(256), 0, st,
-                               s->n, n, s->dt, s->cnt.p, s->chunk0.p,
-                               s->stride.p, s->brk.p, s->jint.p, s->flags.p,
-                               s->r0.p, s->seg.p + (size_t)(n - n0) * B,
-                               s->vol.p, cfield, s->lam[cur].p,
-                               s->lam[1 - cur].p, s->gradc.p);
+            hipLaunchKernelGGL(k_adj_pack, dim3(pblocks), dim3(256), 0, st,
+                               s->n, s->r0.p, s->seg.p + (size_t)(n - n0) * B,
+                               s->vol.p, s->cnode.p, s->flags.p,
+                               s->lam[cur].p, s->arec.p);
             HIPCHK(hipGetLastError());
+            aa.step = n;
+            aa.lam_out = s->lam[1 - cur].p;
+            if (s->ntiles > 0)
+                HIPCHK(launch_adjoint(s, aa));
             cur = 1 - cur;
         }
     }

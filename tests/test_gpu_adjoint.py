@@ -106,3 +106,28 @@ def test_torch_autograd_function():
         w, w.extra["g"], w.bond_stiffness, solver.steps, sensors, target)
     assert float(L) == pytest.approx(Lr, rel=1e-10)
     assert pc.rel_l2(g.grad.numpy(), grad_r) < 1e-10
+
+
+def test_gradient_at_a_large_step_with_nodes_at_rest():
+    """dt = 1e-5 on the stiff bar: k dt ~ 1e16, nodes away from the pulled
+    face stay exactly at rest while its neighbours fly off and break.  A
+    bond with l == l0 bitwise must get 1/l0 - 1/l = 0 exactly (its forward
+    stretch is 0): the gradient matches central differences."""
+    w = adjoint_workload(shape=(30, 10, 10), dt=1e-5, pull=1e-8, seed=4)
+    rng = np.random.default_rng(4)
+    sensors = rng.choice(w.nnodes, 24, replace=False)
+    solver = adjoint.StiffnessAdjoint(w, 30, checkpoint_every=8)
+    g = w.extra["g"]
+    target = solver.forward(0.5 * g)[sensors]
+    L, grad, u = _loss_grad(solver, g, sensors, target)
+    _, dmg, _ = solver.state()
+    assert np.all(np.isfinite(grad)) and 0 < np.mean(dmg) < 0.5
+    h = 1e-6
+    worst = 0.0
+    for _ in range(4):
+        v = rng.standard_normal(w.nnodes)
+        Lp = _loss_grad(solver, g + h * v, sensors, target)[0]
+        Lm = _loss_grad(solver, g - h * v, sensors, target)[0]
+        fd = (Lp - Lm) / (2 * h)
+        worst = max(worst, abs(grad @ v - fd) / abs(fd))
+    assert worst < 1e-6
