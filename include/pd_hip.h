@@ -119,6 +119,19 @@ int pd_set_node_stiffness(pd_sim *sim, const double *c);
  * (cl/peridynamics.cl:78). */
 int pd_set_break_mode(pd_sim *sim, int tension_only);
 
+/* Exact-order step of the Cython path (PD_LAYOUT_COMPACT, PD_EULER, scalar
+ * stiffness, no corrections): 1 sums each node's bond forces in the order
+ * of peridynamics.pyx:34-99 (bond_force) -- the live neighbours k below the
+ * node in ascending k, then those above it in the node's current row order,
+ * then the force boundary condition -- with the reference's operations, so
+ * that force and u (peridynamics.pyx:159-185) equal the Cython path's
+ * bitwise, not only to rounding.  A verification mode: the force runs one
+ * thread per node with global reads, several times slower than the default
+ * step.  Assumes symmetric neighbour lists, as the model builds them.  0
+ * (default) turns it off.  No counterpart in the reference (it replaces the
+ * whole of Euler.__call__, integrators.py:330-349, bit for bit). */
+int pd_set_exact_order(pd_sim *sim, int on);
+
 /* Launch shape of the step kernel: lanes per node (1, 2, 4 or 8) and the
  * largest number of nodes per tile (a multiple of 64, with
  * tile_nodes * lanes_per_node <= 1024; one workgroup runs one tile).

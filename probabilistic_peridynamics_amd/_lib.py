@@ -46,6 +46,7 @@ SIGNATURES = {
         _I]),
     "pd_set_node_stiffness": (ctypes.c_int, [_SIM, _D]),
     "pd_set_break_mode": (ctypes.c_int, [_SIM, ctypes.c_int]),
+    "pd_set_exact_order": (ctypes.c_int, [_SIM, ctypes.c_int]),
     "pd_set_launch": (ctypes.c_int, [_SIM, ctypes.c_int, ctypes.c_int]),
     "pd_set_groups": (ctypes.c_int, [_SIM, _I]),
     "pd_step": (ctypes.c_int, [_SIM, ctypes.c_double, ctypes.c_double]),

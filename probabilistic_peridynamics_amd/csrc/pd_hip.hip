@@ -171,6 +171,7 @@ struct StepArgs {
     const double *cnode;  // node stiffness c (DM_FIELD)
     int *brk;  // if set: the step at which each slot's bond broke
     int step;  // index of this step (for brk)
+    int exact;  // k_exact_euler does the force and update of this step
 };
 
 __host__ __device__ __forceinline__ size_t slot_of(long long chunk0,
@@ -612,6 +613,8 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     if (sub == 0 && a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
         compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
                     a.n_neigh, (uint32_t)a.tnode[i], cnt);
+    if (a.exact)
+        return;  // breaks and lists only: k_exact_euler follows
 
     // the node's lanes share its three components (after the xor reduction
     // every lane holds the sums), so the per-node loads and stores coalesce
@@ -887,6 +890,108 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
     }
     if (threadIdx.x == 0)
         atomicAdd(total, part[0]);
+}
+
+// --- exact-order Euler step (verification mode of the Cython path) --------
+//
+// peridynamics.pyx:34-99 (bond_force) visits i ascending and, for each live
+// j > i of i's row in row order, adds f V_j to F_i and subtracts f V_i from
+// F_j.  So F_m is the left fold, from +0.0, of first the bonds to live
+// k < m in ascending k (subtracting (fn (r_m - r_k)) V_k), then the bonds
+// to the live j > m of m's current row in row order (adding
+// (fn (r_j - r_m)) V_j), then the force boundary condition.  Each thread
+// folds one node in that order with the reference's operations (IEEE square
+// roots and divisions, no contraction): after k_step has broken the bonds
+// and compacted the rows (peridynamics.pyx:102-156), the force and the
+// Euler update (:159-185) equal the Cython path's bitwise.  Slots hold each
+// node's bonds ascending by j (caller ids), so the lower neighbours come
+// first; inv maps caller ids to internal ones.  Assumes symmetric lists, as
+// the model builds them.
+__device__ __forceinline__ double cython_fn(const double *ri,
+                                            const double *rj,
+                                            const double *r0i,
+                                            const double *r0j, double k)
+{
+    // spatial.pyx:20-31 (ceuclid) and :87-97 (cstrain2)
+    double d0 = rj[0] - ri[0], d1 = rj[1] - ri[1], d2 = rj[2] - ri[2];
+    d0 = d0 * d0;
+    d1 = d1 * d1;
+    d2 = d2 * d2;
+    const double l = sqrt((d0 + d1) + d2);
+    double e0 = r0j[0] - r0i[0], e1 = r0j[1] - r0i[1], e2 = r0j[2] - r0i[2];
+    e0 = e0 * e0;
+    e1 = e1 * e1;
+    e2 = e2 * e2;
+    const double l0 = sqrt((e0 + e1) + e2);
+    const double strain = (l - l0) / l0;
+    // peridynamics.pyx:75-80
+    double fn = strain * k;
+    fn = fn / l;
+    return fn;
+}
+
+__global__ void k_exact_euler(StepArgs a, const int *inv, const double *vol,
+                              double k, double dmag, double fmag)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n)
+        return;
+    const uint32_t m = (uint32_t)a.tnode[p];
+    const NodeRec own = a.rec_old[p];
+    const double rm[3] = {own.b.x, own.b.y, own.b.z};
+    const double r0m[3] = {own.a.x, own.a.y, own.a.z};
+    double F[3] = {0.0, 0.0, 0.0};
+    const int c = a.cnt[p];
+    for (int q = 0; q < c; ++q) {
+        const uint32_t e =
+            (uint32_t)a.gidx[slot_of(a.chunk0[p], a.stride[p], q)];
+        const uint32_t key = e & KEYM;
+        if (key >= m)
+            break;
+        if (e & DEAD)
+            continue;
+        const int g = inv[key];
+        const NodeRec o = a.rec_old[g];
+        const double rk[3] = {o.b.x, o.b.y, o.b.z};
+        const double r0k[3] = {o.a.x, o.a.y, o.a.z};
+        const double fn = cython_fn(rk, rm, r0k, r0m, k);
+        const double vk = vol[g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            F[d] = F[d] - (fn * (rm[d] - rk[d])) * vk;
+    }
+    const int *row = a.nlist_ref + (size_t)m * a.K;
+    const int nn = a.n_neigh[m];
+    for (int t = 0; t < nn; ++t) {
+        const int j = row[t];
+        if (j <= (int)m)
+            continue;
+        const int g = inv[j];
+        const NodeRec o = a.rec_old[g];
+        const double rj[3] = {o.b.x, o.b.y, o.b.z};
+        const double r0j[3] = {o.a.x, o.a.y, o.a.z};
+        const double fn = cython_fn(rm, rj, r0m, r0j, k);
+        const double vj = vol[g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            F[d] = F[d] + (fn * (rj[d] - rm[d])) * vj;
+    }
+    const int fl = a.flags[p];
+    const size_t b3 = 3 * (size_t)p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const size_t kd = b3 + d;
+        // peridynamics.pyx:93-97, then :179-185
+        const bool fbc = (fl >> (3 + d)) & 1;
+        const double Fd = fbc ? F[d] + fmag * a.fbcv[kd] : F[d];
+        if (fbc)
+            a.body_force[kd] = F[d];
+        a.force[kd] = Fd;
+        const double u =
+            (fl >> d) & 1 ? dmag * a.bcv[kd] : a.u[kd] + a.dt * Fd;
+        a.u[kd] = u;
+        reinterpret_cast<double *>(&a.rec_new[p].b)[d] = r0m[d] + u;
+    }
 }
 
 // --- adjoint of the Euler step with a recorded break mask -------------------
@@ -1167,6 +1272,8 @@ struct pd_sim {
     int nregimes = 1, nbond_types = 1;
     double k = 0.0, s0 = 0.0;
     int cur = 0;
+    bool exact = false;  // exact-order Euler step (k_exact_euler)
+    DevBuf<int> inv;     // caller id -> internal id, for the exact step
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -1335,6 +1442,7 @@ StepArgs make_args(pd_sim *s)
     a.cnode = s->cnode.p;
     a.brk = s->adj_recording ? s->brk.p : nullptr;
     a.step = s->step_counter;
+    a.exact = s->exact ? 1 : 0;
     return a;
 }
 
@@ -1457,6 +1565,12 @@ int launch_step(pd_sim *s, double dmag, double fmag)
         s->cur = 1 - s->cur;
         return PD_OK;
     }
+    if (s->exact && (s->layout != PD_LAYOUT_COMPACT ||
+                     s->integrator != PD_EULER || dm_of(s) != DM_PMB ||
+                     s->has_corr || !s->inv.p))
+        return fail(PD_E_STATE, "the exact-order step needs the compact "
+                                "layout, the Euler integrator and the scalar "
+                                "model without corrections");
     const StepArgs a = make_args(s);
     hipError_t e;
     switch (dm_of(s)) {
@@ -1469,6 +1583,12 @@ int launch_step(pd_sim *s, double dmag, double fmag)
     default:
         e = launch_dm<DM_PMB>(s, a, dmag, fmag);
         break;
+    }
+    if (e == hipSuccess && s->exact) {
+        hipLaunchKernelGGL(k_exact_euler, dim3((s->n + 255) / 256),
+                           dim3(256), 0, s->stream, a, s->inv.p, s->vol.p,
+                           s->k, dmag, fmag);
+        e = hipGetLastError();
     }
     if (e != hipSuccess)
         return fail(PD_E_HIP, std::string("step kernel launch: ") +
@@ -1490,6 +1610,16 @@ int upload_tips(pd_sim *s)
         q[e] = 3 * inv[s->h_tip_node[e]] + s->h_tip_dof[e];
     HIPCHK(upload(s->tip_q, q.data(), q.size()));
     HIPCHK(upload(s->tip_off, s->h_tip_off.data(), s->h_tip_off.size()));
+    return PD_OK;
+}
+
+// inv[caller id] = internal id for the current tiling.
+int upload_inv(pd_sim *s)
+{
+    std::vector<int> inv(s->n);
+    for (int p = 0; p < s->n; ++p)
+        inv[s->perm[p]] = p;
+    HIPCHK(upload(s->inv, inv.data(), inv.size()));
     return PD_OK;
 }
 
@@ -2074,6 +2204,11 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     rc = upload_tips(s);
     if (rc)
         return rc;
+    if (s->exact) {
+        rc = upload_inv(s);
+        if (rc)
+            return rc;
+    }
     s->has_buffers = true;
     s->retile_pending = false;
     adjoint_release(s);  // slot arrays were rebuilt
@@ -2194,6 +2329,28 @@ int pd_set_break_mode(pd_sim *s, int tension_only)
     if (rc)
         return rc;
     s->tension_only = tension_only ? 1 : 0;
+    return PD_OK;
+}
+
+int pd_set_exact_order(pd_sim *s, int on)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!on) {
+        s->exact = false;
+        s->inv.release();
+        return PD_OK;
+    }
+    if (s->layout != PD_LAYOUT_COMPACT || s->integrator != PD_EULER)
+        return fail(PD_E_ARG, "the exact-order step is the Cython path's: "
+                              "PD_LAYOUT_COMPACT and PD_EULER only");
+    if (s->has_buffers && !s->perm.empty()) {
+        rc = upload_inv(s);
+        if (rc)
+            return rc;
+    }
+    s->exact = true;
     return PD_OK;
 }
 

@@ -431,13 +431,19 @@ class Euler(Integrator):
     _kind = PD_EULER
     _layout = PD_LAYOUT_COMPACT
 
-    def __init__(self, dt, device=0):
-        """Create an :class:`Euler` integrator for HIP device ``device``."""
+    def __init__(self, dt, device=0, exact_order=False):
+        """Create an :class:`Euler` integrator for HIP device ``device``.
+
+        ``exact_order=True`` sums each node's bond forces in the Cython
+        path's order (peridynamics.pyx:34-99), so that force and u equal it
+        bitwise: a slower verification mode (see :meth:`set_exact_order`).
+        """
         self.dt = dt
         self.context = None
         self._device = int(device)
         self._sim = None
         self.damping = 0.0
+        self.exact_order = bool(exact_order)
 
     def __call__(self, displacement_bc_magnitude, force_bc_magnitude):
         """One step: break bonds, bond force, Euler update."""
@@ -472,6 +478,16 @@ class Euler(Integrator):
         super().build(nnodes, degrees_freedom, max_neighbours, coords, volume,
                       family, bc_types, bc_values, force_bc_types,
                       force_bc_values, None, None, None)
+        if self.exact_order:
+            self.set_exact_order(True)
+
+    def set_exact_order(self, on):
+        """Exact-order step on (True) or off: the default step sums a
+        node's bonds in its own order and matches the Cython path to
+        rounding; the exact-order step reproduces it bitwise."""
+        self._need_sim()
+        check(_lib.lib().pd_set_exact_order(self._sim, int(bool(on))))
+        self.exact_order = bool(on)
 
     def create_buffers(self, nlist, n_neigh, bond_stiffness,
                        critical_stretch, plus_cs, u, ud, udd, force,
