@@ -52,18 +52,29 @@ def read_array(read_path, dataset):
 
 
 def vectorised(fn):
-    """Mark a set-up callback (``is_bond_type``, ``is_density``) as one that
-    works on many nodes at once.
+    """Mark a set-up callback as one that works on many nodes at once.
 
     The reference calls ``is_bond_type(x, y)`` once per bond and
-    ``is_density(x)`` once per node from Python (model.py:657-780): 1e8
-    calls on a million-node mesh.  A function marked with this decorator is
-    called instead with coordinate arrays of shape (3, M) -- component
-    first, so ``x[0]`` is the x coordinate of every point, as for a single
-    node -- and must return an array of M results: integer bond types for
-    ``is_bond_type``, float densities for ``is_density``.  Bonds are passed
-    in the reference's order (node i ascending, then i's row order) in
-    blocks of at most 4M bonds.  The range checks are the reference's.
+    ``is_density(x)``, ``is_displacement_boundary(x)``,
+    ``is_force_boundary(x)`` and ``is_tip(x)`` once per node from Python
+    (model.py:657-780, 1093-1212): 1e8 calls on a million-node mesh.  A
+    function marked with this decorator is called instead with coordinate
+    arrays of shape (3, M) -- component first, so ``x[0]`` is the x
+    coordinate of every point, as for a single node -- and must return the
+    results for all M points at once:
+
+    * ``is_bond_type``: M integer bond types.  Bonds are passed in the
+      reference's order (node i ascending, then i's row order) in blocks of
+      at most 4M bonds; the range checks are the reference's.
+    * ``is_density``: M float densities.
+    * ``is_displacement_boundary``, ``is_force_boundary``: a float array of
+      shape (3, M), the value of DOF d of point m at [d, m], NaN where the
+      per-node function would return None.
+    * ``is_tip``: an array of shape (3, M) (any array-like; it is read as
+      objects) holding what the per-node function would return per DOF:
+      None, a tip name, or a tuple of names.
+
+    The arrays built from them equal those of the per-node calls.
     """
     fn.vectorised = True
     return fn
@@ -71,6 +82,18 @@ def vectorised(fn):
 
 def _is_vectorised(fn):
     return getattr(fn, "vectorised", False) is True
+
+
+@vectorised
+def _no_condition(x):
+    """No boundary condition anywhere (the default callbacks)."""
+    return np.full((3, np.shape(x)[1]), np.nan)
+
+
+@vectorised
+def _no_tip(x):
+    """No tip anywhere."""
+    return np.full((3, np.shape(x)[1]), None, dtype=object)
 
 
 def create_crack(crack, nlist, n_neigh):
@@ -276,14 +299,11 @@ class Model(object):
         self.densities = self._set_densities(density, is_density)
 
         if is_force_boundary is None:
-            def is_force_boundary(x):
-                return [None, None, None]
+            is_force_boundary = _no_condition
         if is_displacement_boundary is None:
-            def is_displacement_boundary(x):
-                return [None, None, None]
+            is_displacement_boundary = _no_condition
         if is_tip is None:
-            def is_tip(x):
-                return [None, None, None]
+            is_tip = _no_tip
 
         (self.bc_types, self.bc_values, self.force_bc_types,
          self.force_bc_values, self.tip_types,
@@ -691,6 +711,8 @@ class Model(object):
         for name, fn in functions.items():
             if not callable(fn):
                 raise TypeError("{} must be a *function*.".format(name))
+            if _is_vectorised(fn):
+                continue
             if type(fn(self.coords[0][:])) is not list:
                 raise TypeError(
                     "{} must be a function that returns a *list*.".format(
@@ -715,32 +737,76 @@ class Model(object):
             ntips[key] += 1
             tip_types[key].append((i, j))
 
-        for i in range(self.nnodes):
-            bnd = is_displacement_boundary(self.coords[i][:])
-            fbnd = is_force_boundary(self.coords[i][:])
-            tip = is_tip(self.coords[i][:])
+        # the vectorised callbacks, all nodes at once (NaN / None: no entry)
+        xt = self.coords.T.copy()
+        vec_bnd = vec_fbnd = vec_tip = None
+        if _is_vectorised(is_displacement_boundary):
+            vec_bnd = self._vector_conditions(is_displacement_boundary,
+                                              "is_displacement_boundary", xt)
+            set_ = ~np.isnan(vec_bnd)
+            bc_types[set_] = 1
+            bc_values[set_] = vec_bnd[set_]
+        if _is_vectorised(is_force_boundary):
+            vec_fbnd = self._vector_conditions(is_force_boundary,
+                                               "is_force_boundary", xt)
+            set_ = ~np.isnan(vec_fbnd)
+            force_bc_types[set_] = 1
+            force_bc_values[set_] = (vec_fbnd
+                                     / self.volume[:, np.newaxis])[set_]
+            num_force_bc_nodes = int(np.count_nonzero(set_.any(axis=1)))
+        if _is_vectorised(is_tip):
+            vec_tip = np.asarray(is_tip(xt), dtype=object)
+            if vec_tip.shape != (3, self.nnodes):
+                raise TypeError(
+                    "a vectorised is_tip must return an array of shape "
+                    "(3, nnodes) (got shape {})".format(vec_tip.shape))
+            vec_tip = vec_tip.T
+        per_node = vec_bnd is None or vec_fbnd is None or vec_tip is None
+        for i in range(self.nnodes if per_node else 0):
+            bnd = (None if vec_bnd is not None
+                   else is_displacement_boundary(self.coords[i][:]))
+            fbnd = (None if vec_fbnd is not None
+                    else is_force_boundary(self.coords[i][:]))
+            tip = None if vec_tip is not None else is_tip(self.coords[i][:])
             is_force_node = 0
             for j in range(3):
-                if bnd[j] is not None:
+                if bnd is not None and bnd[j] is not None:
                     bc_types[i, j] = np.intc(1)
                     bc_values[i, j] = np.float64(bnd[j])
-                if fbnd[j] is not None:
+                if fbnd is not None and fbnd[j] is not None:
                     is_force_node = 1
                     force_bc_types[i, j] = np.intc(1)
                     force_bc_values[i, j] = np.float64(
                         fbnd[j] / self.volume[i])
-                if tip[j] is not None:
+                if tip is not None and tip[j] is not None:
                     if type(tip[j]) is tuple:
                         for t in tip[j]:
                             set_tip(t, i, j)
                     else:
                         set_tip(tip[j], i, j)
             num_force_bc_nodes += is_force_node
+        if vec_tip is not None:
+            # the per-node loop's order: node ascending, then DOF
+            for i, j in zip(*np.nonzero(vec_tip != None)):  # noqa: E711
+                t = vec_tip[i, j]
+                for name in (t if type(t) is tuple else (t,)):
+                    set_tip(name, int(i), int(j))
         if num_force_bc_nodes != 0:
             force_bc_values = np.float64(
                 np.divide(force_bc_values, num_force_bc_nodes))
         return (bc_types, bc_values, force_bc_types, force_bc_values,
                 tip_types, ntips)
+
+    def _vector_conditions(self, fn, name, xt):
+        """(nnodes, 3) values of a vectorised boundary-condition callback,
+        NaN where there is no condition."""
+        v = np.asarray(fn(xt))
+        if v.shape != (3, self.nnodes) or v.dtype.kind != "f":
+            raise TypeError(
+                "a vectorised {} must return a float array of shape "
+                "(3, nnodes), NaN for no condition (got {} of shape "
+                "{})".format(name, v.dtype, v.shape))
+        return v.T.astype(np.float64)
 
     # ------------------------------------------------------------------
     # simulation
