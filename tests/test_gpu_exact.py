@@ -157,3 +157,49 @@ def test_switching_and_rejection():
     slot = configs.make_integrator(ws)
     with pytest.raises(ValueError, match="PD_LAYOUT_COMPACT"):
         _lib.check(_lib.lib().pd_set_exact_order(slot._sim, 1))
+
+
+def test_model_simulate_exact_order_bitwise(golden, tmp_path):
+    """The whole model path of the reference's regression test
+    (test_regression.py:36-79): Model with the Cython-layout Euler in
+    exact-order mode, the pre-cracked example mesh, a vectorised
+    displacement boundary, writes every 5 steps; u, force, the lists and
+    damage equal the oracle stepped on the model's own arrays bitwise."""
+    import warnings
+    from probabilistic_peridynamics_amd.model import (Model,
+                                                      initial_crack_helper,
+                                                      vectorised)
+
+    @vectorised
+    def disp_bc(x):
+        out = np.full((3, x.shape[1]), np.nan)
+        lo = x[0] < 1.5 * 0.1
+        hi = ~lo & (x[0] > 1.0 - 1.5 * 0.1)
+        out[:, lo] = np.array([-1.0, 0.0, 0.0])[:, None]
+        out[:, hi] = np.array([1.0, 0.0, 0.0])[:, None]
+        return out
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        m = Model(golden / "example_mesh.vtk",
+                  it.Euler(pc.DT, exact_order=True), horizon=pc.HORIZON,
+                  critical_stretch=pc.CRITICAL_STRETCH,
+                  bond_stiffness=pc.BOND_STIFFNESS,
+                  is_displacement_boundary=disp_bc,
+                  initial_crack=initial_crack_helper(pc.is_crack))
+    nl0, nn0 = (a.copy() for a in m.initial_connectivity)
+    ref = oracle.CythonEuler(pc.DT, m.coords, m.volume, m.family,
+                             m.bc_types, m.bc_values, m.force_bc_types,
+                             m.force_bc_values, nl0, nn0, pc.BOND_STIFFNESS,
+                             pc.CRITICAL_STRETCH)
+    (tmp_path / "w").mkdir()
+    u, damage, (nl, nn), force, _, _ = m.simulate(
+        pc.STEPS, displacement_bc_magnitudes=pc.DISP_MAGS, write=5,
+        write_path=tmp_path / "w")
+    for s in range(pc.STEPS):
+        ref(pc.DISP_MAGS[s], 0.0)
+    np.testing.assert_array_equal(nl, ref.nlist)
+    np.testing.assert_array_equal(nn, ref.n_neigh)
+    np.testing.assert_array_equal(damage, ref.damage())
+    assert np.array_equal(u, ref.u), pc.rel_l2(u, ref.u)
+    assert np.array_equal(force, ref.force), pc.rel_l2(force, ref.force)
