@@ -1,7 +1,8 @@
 """Edge cases of the step kernel against the oracle, through the C ABI: the
 smallest mesh (one bond), a very ragged one (a node with a thousand bonds
 beside nodes with one), a row over the LDS halo capacity (rejected cleanly),
-and a step in which every bond breaks."""
+and a step in which every bond breaks.  "compact_exact" is the Cython layout
+in its exact-order mode, held to the oracle bitwise."""
 import numpy as np
 import pytest
 
@@ -47,7 +48,8 @@ def _run(layout, coords, rows, steps, bc_types, bc_values, dmags, k=1e12,
     vol = np.full(n, 1e-6)
     fbt = np.zeros((n, 3), np.int32)
     fbv = np.zeros((n, 3))
-    integ = it.EulerHIP(dt) if layout == "slot" else it.Euler(dt)
+    integ = (it.EulerHIP(dt) if layout == "slot" else
+             it.Euler(dt, exact_order=layout == "compact_exact"))
     integ.build(n, 3, nl.shape[1], coords, vol, fam, bc_types, bc_values,
                 fbt, fbv, None, None, None)
     z = np.zeros((n, 3))
@@ -71,16 +73,18 @@ def _run(layout, coords, rows, steps, bc_types, bc_values, dmags, k=1e12,
     return got, (ref.u, ref.nlist, rn, rd)
 
 
-def _check(got, ref):
+def _check(got, ref, exact=False):
     u, _, _, _, _, dmg, nl, nn = got
     ru, rnl, rnn, rdmg = ref
     np.testing.assert_array_equal(nl, rnl)
     np.testing.assert_array_equal(nn, rnn)
     np.testing.assert_array_equal(dmg, rdmg)
     assert pc.rel_l2(u, ru) < 1e-10
+    if exact:
+        assert np.array_equal(u, ru)
 
 
-@pytest.mark.parametrize("layout", ["slot", "compact"])
+@pytest.mark.parametrize("layout", ["slot", "compact", "compact_exact"])
 def test_one_bond(layout):
     """Two nodes, one bond, one end held, the other pulled until the bond
     breaks (its stretch reaches 0.01 = s0 exactly, in rounding, at step 6)."""
@@ -90,11 +94,11 @@ def test_one_bond(layout):
     bcv[1, 0] = 1.0
     d = 2e-6 * np.arange(1, 21)
     got, ref = _run(layout, coords, [[1], [0]], 20, bct, bcv, d)
-    _check(got, ref)
+    _check(got, ref, layout == "compact_exact")
     assert got[7][0] == 0 and got[5][0] == 1.0
 
 
-@pytest.mark.parametrize("layout", ["slot", "compact"])
+@pytest.mark.parametrize("layout", ["slot", "compact", "compact_exact"])
 def test_ragged_star(layout):
     """A hub bonded to 1000 nodes on a rough shell, each of those bonded to
     the hub only: one row of 1000 next to rows of 1 (tiles cut by the halo
@@ -112,7 +116,7 @@ def test_ragged_star(layout):
     bcv[0] = [1.0, 0.5, -0.25]
     dm = 3e-6 * np.arange(1, 16)
     got, ref = _run(layout, coords, rows, 15, bct, bcv, dm)
-    _check(got, ref)
+    _check(got, ref, layout == "compact_exact")
     broken = m - got[7][0]
     assert 0 < broken < m
 
@@ -145,7 +149,7 @@ def test_row_over_halo_capacity_is_rejected():
     _check(got, ref)
 
 
-@pytest.mark.parametrize("layout", ["slot", "compact"])
+@pytest.mark.parametrize("layout", ["slot", "compact", "compact_exact"])
 def test_every_bond_breaks_in_the_first_step(layout):
     """An initial displacement that stretches every bond by 50 %: every
     bond breaks in step 1, every node's damage is 1, and no force acts."""
@@ -157,6 +161,6 @@ def test_every_bond_breaks_in_the_first_step(layout):
     bct = np.zeros((n, 3), np.int32)
     got, ref = _run(layout, w_coords, rows, 2, bct, np.zeros((n, 3)),
                     np.zeros(2), u0=u0)
-    _check(got, ref)
+    _check(got, ref, layout == "compact_exact")
     assert np.all(got[7] == 0) and np.all(got[5] == 1.0)
     assert np.all(got[3] == 0.0)
