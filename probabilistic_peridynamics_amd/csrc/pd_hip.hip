@@ -155,7 +155,9 @@ struct StepArgs {
     NodeRec *rec_new;
     const TileInfo *tiles;
     const int *tnode, *halo;
-    double *u, *ud, *udd, *force, *body_force;
+    const double *u;  // u_n
+    double *u_out;    // u_{n+1}: u itself, or the next slot of a replay
+    double *ud, *udd, *force, *body_force;
     const int *flags;
     const double *bcv, *fbcv, *dens;
     const double *dens1;  // per-node density when all three DOFs share it
@@ -660,7 +662,7 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
             a.udd[k] = udd1;
             u = fixed ? dmag * a.bcv[k] : u + a.dt * (ud + hdt * udd1);
         }
-        a.u[k] = u;
+        a.u_out[k] = u;
         // r = r0 + u for the next step; the record's c (w) is static
         const double r0d = d == 0 ? r0i.x : (d == 1 ? r0i.y : r0i.z);
         reinterpret_cast<double *>(&a.rec_new[i].b)[d] = r0d + u;
@@ -989,7 +991,7 @@ __global__ void k_exact_euler(StepArgs a, const int *inv, const double *vol,
         a.force[kd] = Fd;
         const double u =
             (fl >> d) & 1 ? dmag * a.bcv[kd] : a.u[kd] + a.dt * Fd;
-        a.u[kd] = u;
+        a.u_out[kd] = u;
         reinterpret_cast<double *>(&a.rec_new[p].b)[d] = r0m[d] + u;
     }
 }
@@ -1102,10 +1104,15 @@ struct AdjArgs {
     double dt;
     const TileInfo *tiles;
     const int *halo, *cnt, *brk;
-    const NodeRec *rec;  // k_adj_pack's records of this step
+    const NodeRec *rec;  // the records of this step
     const unsigned short *loc0;  // loc at the start of the recording
     const double *il0;
     double *lam_out, *gradc;
+    // if set, the records of the step before (k_adj_pack's, from u_prev =
+    // u_{n-1} and lambda_n) are written too
+    NodeRec *rec_next;
+    const double *u_prev, *r0;
+    const int *flags;
 };
 
 // One backward step on the step kernel's tiles: the tile's halo records
@@ -1212,8 +1219,23 @@ __global__ __launch_bounds__(1024) void k_adj_tile(AdjArgs a)
     if (!valid)
         return;
     const size_t b = 3 * (size_t)i;
-    for (int d = sub; d < 3; d += LPN)
-        a.lam_out[b + d] = mum[d] + a.dt * jt[d];
+    const int fl = a.rec_next ? a.flags[i] : 0;
+    for (int d = sub; d < 4; d += LPN) {
+        if (d < 3) {
+            const double lam = mum[d] + a.dt * jt[d];
+            a.lam_out[b + d] = lam;
+            if (a.rec_next) {
+                // as k_adj_pack: r = r0 + u_{n-1}, mu = lambda_n masked
+                double *ra = reinterpret_cast<double *>(&a.rec_next[i].a);
+                double *rb = reinterpret_cast<double *>(&a.rec_next[i].b);
+                ra[d] = a.r0[b + d] + a.u_prev[b + d];
+                rb[d] = (fl >> d) & 1 ? 0.0 : lam;
+            }
+        } else if (a.rec_next) {
+            reinterpret_cast<double *>(&a.rec_next[i].a)[3] = vm;
+            reinterpret_cast<double *>(&a.rec_next[i].b)[3] = cm;
+        }
+    }
     if (sub == 0)
         a.gradc[i] += a.dt * gc;
 }
@@ -1273,6 +1295,9 @@ struct pd_sim {
     double k = 0.0, s0 = 0.0;
     int cur = 0;
     bool exact = false;  // exact-order Euler step (k_exact_euler)
+    // a replay's u_n and u_{n+1} (slots of the segment buffer); null: u
+    const double *u_in = nullptr;
+    double *u_out = nullptr;
     DevBuf<int> inv;     // caller id -> internal id, for the exact step
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1304,7 +1329,7 @@ struct pd_sim {
     DevBuf<int> brk;
     DevBuf<unsigned short> loc0;
     DevBuf<double> il0;    // 1/l0 per slot
-    DevBuf<NodeRec> arec;  // per-step records of the backward kernel
+    DevBuf<NodeRec> arec[2];  // records of a backward step, ping-pong
     DevBuf<double> ckpt, seg, lam[2], gradc, fin_u, fin_force, fin_bforce;
 
     // monitored sums: tip entries (caller's node, dof) grouped, in order
@@ -1417,7 +1442,8 @@ StepArgs make_args(pd_sim *s)
     a.tiles = s->tiles.p;
     a.tnode = s->tnode.p;
     a.halo = s->halo.p;
-    a.u = s->u.p;
+    a.u = s->u_in ? s->u_in : s->u.p;
+    a.u_out = s->u_out ? s->u_out : s->u.p;
     a.ud = s->ud.p;
     a.udd = s->udd.p;
     a.force = s->force.p;
@@ -1519,7 +1545,8 @@ void adjoint_release(pd_sim *s)
     s->brk.release();
     s->loc0.release();
     s->il0.release();
-    s->arec.release();
+    s->arec[0].release();
+    s->arec[1].release();
     s->ckpt.release();
     s->seg.release();
     s->lam[0].release();
@@ -2517,7 +2544,8 @@ int pd_adjoint_start(pd_sim *s, int nsteps, int checkpoint_every)
     HIPCHK(upload(inv_d, inv.data(), N));
     HIPCHK(s->brk.alloc(S));
     HIPCHK(s->il0.alloc(S));
-    HIPCHK(s->arec.alloc(N));
+    HIPCHK(s->arec[0].alloc(N));
+    HIPCHK(s->arec[1].alloc(N));
     HIPCHK(s->loc0.alloc(S));
     HIPCHK(s->ckpt.alloc((size_t)nck * B));
     HIPCHK(s->seg.alloc((size_t)checkpoint_every * B));
@@ -2605,13 +2633,19 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
     aa.halo = s->halo.p;
     aa.cnt = s->cnt.p;
     aa.brk = s->brk.p;
-    aa.rec = s->arec.p;
+    aa.r0 = s->r0.p;
+    aa.flags = s->flags.p;
     aa.loc0 = s->loc0.p;
     aa.il0 = s->il0.p;
     aa.gradc = s->gradc.p;
     struct ReplayGuard {  // steps below are replays, also on early returns
         pd_sim *s;
-        ~ReplayGuard() { s->adj_replay = false; }
+        ~ReplayGuard()
+        {
+            s->adj_replay = false;
+            s->u_in = nullptr;
+            s->u_out = nullptr;
+        }
     } guard{s};
     s->adj_replay = true;
     for (int k = nck - 1; k >= 0 && rc == PD_OK; --k) {
@@ -2624,26 +2658,37 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
         HIPCHK(hipMemcpyAsync(s->seg.p, s->u.p, BB, hipMemcpyDeviceToDevice,
                               st));
         // replay the segment: the same kernel from the same state reproduces
-        // the recorded steps (and their breaks) bitwise
+        // the recorded steps (and their breaks) bitwise; step n reads u_n
+        // from slot n - n0 of the segment and writes u_{n+1} to the next
         for (int n = n0; n < n1 - 1 && rc == PD_OK; ++n) {
             s->step_counter = n;
+            s->u_in = s->seg.p + (size_t)(n - n0) * B;
+            s->u_out = s->seg.p + (size_t)(n - n0 + 1) * B;
             rc = launch_step(s, s->adj_dmag[n], s->adj_fmag[n]);
-            if (rc == PD_OK)
-                HIPCHK(hipMemcpyAsync(s->seg.p + (size_t)(n - n0 + 1) * B,
-                                      s->u.p, BB, hipMemcpyDeviceToDevice,
-                                      st));
         }
-        for (int n = n1 - 1; n >= n0 && rc == PD_OK; --n) {
-            hipLaunchKernelGGL(k_adj_pack, dim3(pblocks), dim3(256), 0, st,
-                               s->n, s->r0.p, s->seg.p + (size_t)(n - n0) * B,
-                               s->vol.p, s->cnode.p, s->flags.p,
-                               s->lam[cur].p, s->arec.p);
-            HIPCHK(hipGetLastError());
+        s->u_in = nullptr;
+        s->u_out = nullptr;
+        if (rc)
+            break;
+        // the records of the segment's last step; each backward step then
+        // writes those of the step before it, within the segment
+        int ra = 0;
+        hipLaunchKernelGGL(k_adj_pack, dim3(pblocks), dim3(256), 0, st, s->n,
+                           s->r0.p, s->seg.p + (size_t)(n1 - 1 - n0) * B,
+                           s->vol.p, s->cnode.p, s->flags.p, s->lam[cur].p,
+                           s->arec[ra].p);
+        HIPCHK(hipGetLastError());
+        for (int n = n1 - 1; n >= n0; --n) {
             aa.step = n;
+            aa.rec = s->arec[ra].p;
             aa.lam_out = s->lam[1 - cur].p;
+            aa.rec_next = n > n0 ? s->arec[1 - ra].p : nullptr;
+            aa.u_prev = n > n0 ? s->seg.p + (size_t)(n - 1 - n0) * B
+                               : nullptr;
             if (s->ntiles > 0)
                 HIPCHK(launch_adjoint(s, aa));
             cur = 1 - cur;
+            ra = 1 - ra;
         }
     }
     s->adj_replay = false;
