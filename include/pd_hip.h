@@ -187,7 +187,7 @@ int pd_run_graph(pd_sim *sim, int nsteps, const double *disp_mags,
  * checkpoint_every steps; the backward pass replays each segment from its
  * checkpoint with the step kernel (bitwise the recorded trajectory) and
  * runs the adjoint kernel (on the step kernel's tiles) back through it.
- * Device memory: 14 bytes per bond slot, and per node 192 bytes plus 24
+ * Device memory: 14 bytes per bond slot, and per node 256 bytes plus 24
  * per checkpoint and per step of a segment.  Requires PD_EULER,
  * PD_LAYOUT_SLOT and pd_set_node_stiffness.  pd_create_buffers or
  * pd_adjoint_stop ends a recording. */
