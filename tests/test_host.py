@@ -259,7 +259,9 @@ def test_create_crack_semantics():
     np.testing.assert_array_equal(nn, [2, 1, 2, 1])
 
 
-def test_workloads_match_survey_sizes():
+def test_bar_workload_sizes():
+    """The 2560-node pre-cracked bar: nodes, bonds after the crack, crack
+    bonds and family range."""
     w = configs.bar_euler()
     assert w.nnodes == 2560 and w.live_bonds() == 219040
     assert w.extra["crack_bonds"] == 1428
