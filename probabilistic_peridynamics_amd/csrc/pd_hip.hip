@@ -1028,6 +1028,19 @@ __global__ void k_adj_restore(int n, int n0, const int *cnt,
     }
 }
 
+// A per-node array in the caller's node order: dst[tnode[p]] = src[p], w
+// values per node.
+__global__ void k_unperm(int n, int w, const int *tnode, const double *src,
+                         double *dst)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const size_t c = (size_t)tnode[p] * w, q = (size_t)p * w;
+    for (int d = 0; d < w; ++d)
+        dst[c + d] = src[q + d];
+}
+
 // brk at the start of a recording: -1 for bonds already broken, else "never".
 __global__ void k_adj_init(int n, const int *cnt, const long long *chunk0,
                            const int *stride, const unsigned short *loc,
@@ -1299,6 +1312,7 @@ struct pd_sim {
     const double *u_in = nullptr;
     double *u_out = nullptr;
     DevBuf<int> inv;     // caller id -> internal id, for the exact step
+    DevBuf<double> wbuf;  // a per-node array reordered for download
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -1389,19 +1403,22 @@ hipError_t upload_perm(DevBuf<T> &d, const std::vector<int> &perm,
 }
 
 // Download rows of w values in tile order into the caller's order.
-template <typename T>
-hipError_t download_perm(T *dst, const DevBuf<T> &d,
-                         const std::vector<int> &perm, int w)
+// n*w doubles of an internal-order device array into dst in the caller's
+// node order: reordered on the device, then one copy.
+int download_caller(pd_sim *s, double *dst, const double *src, int w)
 {
-    std::vector<T> v(perm.size() * (size_t)w);
-    hipError_t e = hipMemcpy(v.data(), d.p, v.size() * sizeof(T),
-                             hipMemcpyDeviceToHost);
-    if (e != hipSuccess)
-        return e;
-    for (size_t p = 0; p < perm.size(); ++p)
-        for (int d2 = 0; d2 < w; ++d2)
-            dst[(size_t)perm[p] * w + d2] = v[p * w + d2];
-    return hipSuccess;
+    const size_t count = (size_t)s->n * w;
+    if (count == 0)
+        return PD_OK;
+    if (s->wbuf.n < count)
+        HIPCHK(s->wbuf.alloc(count));
+    hipLaunchKernelGGL(k_unperm, dim3((s->n + 255) / 256), dim3(256), 0,
+                       s->stream, s->n, w, s->tnode.p, src, s->wbuf.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, s->wbuf.p, count * sizeof(double),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return PD_OK;
 }
 
 int check_sim(pd_sim *s)
@@ -2705,7 +2722,9 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
     if (rc)
         return rc;
     s->step_counter = nsteps;
-    HIPCHK(download_perm(dL_dc, s->gradc, s->perm, 1));
+    rc = download_caller(s, dL_dc, s->gradc.p, 1);
+    if (rc)
+        return rc;
     return PD_OK;
 }
 
@@ -2737,22 +2756,27 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
                        s->damage.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (u)
-        HIPCHK(download_perm(u, s->u, s->perm, 3));
-    if (ud)
-        HIPCHK(download_perm(ud, s->ud, s->perm, 3));
-    if (udd)
-        HIPCHK(download_perm(udd, s->udd, s->perm, 3));
-    if (force)
-        HIPCHK(download_perm(force, s->force, s->perm, 3));
+    const struct {
+        double *dst;
+        const double *src;
+    } fields[] = {{u, s->u.p}, {ud, s->ud.p}, {udd, s->udd.p},
+                  {force, s->force.p}, {body_force, s->body_force.p}};
+    for (const auto &f : fields) {
+        if (f.dst) {
+            rc = download_caller(s, f.dst, f.src, 3);
+            if (rc)
+                return rc;
+        }
+    }
     if (body_force) {
         // stored only on force-BC DOFs; equal to force everywhere else
-        HIPCHK(download_perm(body_force, s->body_force, s->perm, 3));
         std::vector<double> tmp;
         const double *f = force;
         if (!f) {
             tmp.resize(3 * (size_t)N);
-            HIPCHK(download_perm(tmp.data(), s->force, s->perm, 3));
+            rc = download_caller(s, tmp.data(), s->force.p, 3);
+            if (rc)
+                return rc;
             f = tmp.data();
         }
         for (size_t q = 0; q < 3 * (size_t)N; ++q)
