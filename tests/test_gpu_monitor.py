@@ -184,3 +184,26 @@ def test_simulate_in_library_loops_equals_per_step_calls(golden, tmp_path):
     for key in a[5]:
         for k, v in a[5][key].items():
             assert np.array_equal(v, b[5][key][k]), (key, k)
+
+
+def test_partial_writes_equal_the_full_write(golden):
+    """pd_write with some pointers NULL (body_force without force, which
+    takes force where no force BC applies; ud alone; u and damage alone)
+    gives what the full write gives, for the same state."""
+    dptr = _lib.dptr
+    m = _model(golden)
+    m.simulate(4, displacement_bc_magnitudes=1e-4 * np.ones(4),
+               force_bc_magnitudes=np.ones(4))
+    integ = m.integrator
+    full = integ.write(None, None, None, None, None, None, None, None)
+    n = m.nnodes
+    bf = np.full((n, 3), np.nan)
+    ud = np.full((n, 3), np.nan)
+    _lib.check(_lib.lib().pd_write(integ._sim, None, None, None, None,
+                                   dptr(bf), None, None, None))
+    _lib.check(_lib.lib().pd_write(integ._sim, None, dptr(ud), None, None,
+                                   None, None, None, None))
+    assert np.array_equal(bf, full[4]) and np.array_equal(ud, full[1])
+    assert np.any(full[4] != full[3])  # force BCs make them differ
+    u, dmg = integ.write_fields(None, None)
+    assert np.array_equal(u, full[0]) and np.array_equal(dmg, full[5])
