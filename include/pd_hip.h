@@ -123,13 +123,14 @@ int pd_set_break_mode(pd_sim *sim, int tension_only);
  * stiffness, no corrections): 1 sums each node's bond forces in the order
  * of peridynamics.pyx:34-99 (bond_force) -- the live neighbours k below the
  * node in ascending k, then those above it in the node's current row order,
- * then the force boundary condition -- with the reference's operations, so
- * that force and u (peridynamics.pyx:159-185) equal the Cython path's
- * bitwise, not only to rounding.  A verification mode: the force runs one
- * thread per node with global reads, several times slower than the default
- * step.  Assumes symmetric neighbour lists, as the model builds them.  0
- * (default) turns it off.  No counterpart in the reference (it replaces the
- * whole of Euler.__call__, integrators.py:330-349, bit for bit). */
+ * then the force boundary condition -- with the reference's operations in
+ * IEEE double without contraction, so that force and u
+ * (peridynamics.pyx:159-185) equal that arithmetic bitwise, not only to
+ * rounding.  A verification mode: the force runs one thread per node with
+ * global reads, several times slower than the default step.  Assumes
+ * symmetric neighbour lists, as the model builds them.  0 (default) turns
+ * it off.  No counterpart in the reference (it replaces the whole of
+ * Euler.__call__, integrators.py:330-349). */
 int pd_set_exact_order(pd_sim *sim, int on);
 
 /* Launch shape of the step kernel: lanes per node (1, 2, 4 or 8) and the
