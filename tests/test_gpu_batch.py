@@ -122,3 +122,25 @@ def test_autotune_restores_initial_state():
     fresh.run(6, graph=True)
     for a, c in zip(b.results(), fresh.results()):
         np.testing.assert_array_equal(a["u"], c["u"])
+
+
+def test_full_batch_of_256_samples_matches_the_oracle():
+    """The whole batched workload (256 samples, 5,633,380 nodes) for six
+    steps at its configured dt, one graph, every sample against its own
+    oracle run: the broken set, lists and damage exactly, u to 1e-10."""
+    ws = configs.uq_samples()
+    assert len(ws) == 256 and sum(w.nnodes for w in ws) == 5633380
+    b = batch.SampleBatch(ws)
+    b.run(6, graph=True)
+    worst = 0.0
+    broke = 0
+    for w, r in zip(ws, b.results()):
+        ref = _oracle_run(w, 6)
+        nn, dmg = ref.connectivity()
+        np.testing.assert_array_equal(r["nlist"], ref.nlist)
+        np.testing.assert_array_equal(r["n_neigh"], nn)
+        np.testing.assert_array_equal(r["damage"], dmg)
+        worst = max(worst, pc.rel_l2(r["u"], ref.u))
+        broke += int(w.n_neigh.sum() - nn.sum())
+    print(f"256 samples: worst u rel-L2 {worst:.2e}, {broke} bonds broken")
+    assert worst < PARITY_RTOL and broke > 0
