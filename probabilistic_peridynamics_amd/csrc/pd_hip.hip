@@ -943,40 +943,74 @@ __global__ void k_exact_euler(StepArgs a, const int *inv, const double *vol,
     const double rm[3] = {own.b.x, own.b.y, own.b.z};
     const double r0m[3] = {own.a.x, own.a.y, own.a.z};
     double F[3] = {0.0, 0.0, 0.0};
+    // neighbours EB at a time: their index and record loads are issued
+    // together, then they are folded in order (g < 0: not folded)
+    constexpr int EB = 4;
     const int c = a.cnt[p];
-    for (int q = 0; q < c; ++q) {
-        const uint32_t e =
-            (uint32_t)a.gidx[slot_of(a.chunk0[p], a.stride[p], q)];
-        const uint32_t key = e & KEYM;
-        if (key >= m)
-            break;
-        if (e & DEAD)
-            continue;
-        const int g = inv[key];
-        const NodeRec o = a.rec_old[g];
-        const double rk[3] = {o.b.x, o.b.y, o.b.z};
-        const double r0k[3] = {o.a.x, o.a.y, o.a.z};
-        const double fn = cython_fn(rk, rm, r0k, r0m, k);
-        const double vk = vol[g];
+    bool lower = true;
+    for (int q0 = 0; q0 < c && lower; q0 += EB) {
+        int g[EB];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            F[d] = F[d] - (fn * (rm[d] - rk[d])) * vk;
+        for (int t = 0; t < EB; ++t) {
+            const uint32_t e =
+                q0 + t < c ? (uint32_t)a.gidx[slot_of(a.chunk0[p],
+                                                      a.stride[p], q0 + t)]
+                           : KEYM;
+            const uint32_t key = e & KEYM;
+            g[t] = key < m ? ((e & DEAD) ? -1 : inv[key]) : -2;
+        }
+        NodeRec o[EB];
+        double v[EB];
+#pragma unroll
+        for (int t = 0; t < EB; ++t) {
+            if (g[t] >= 0) {
+                o[t] = a.rec_old[g[t]];
+                v[t] = vol[g[t]];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < EB; ++t) {
+            if (g[t] == -2)
+                lower = false;  // keys ascend: the rest are above m
+            if (!lower || g[t] < 0)
+                continue;
+            const double rk[3] = {o[t].b.x, o[t].b.y, o[t].b.z};
+            const double r0k[3] = {o[t].a.x, o[t].a.y, o[t].a.z};
+            const double fn = cython_fn(rk, rm, r0k, r0m, k);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                F[d] = F[d] - (fn * (rm[d] - rk[d])) * v[t];
+        }
     }
     const int *row = a.nlist_ref + (size_t)m * a.K;
     const int nn = a.n_neigh[m];
-    for (int t = 0; t < nn; ++t) {
-        const int j = row[t];
-        if (j <= (int)m)
-            continue;
-        const int g = inv[j];
-        const NodeRec o = a.rec_old[g];
-        const double rj[3] = {o.b.x, o.b.y, o.b.z};
-        const double r0j[3] = {o.a.x, o.a.y, o.a.z};
-        const double fn = cython_fn(rm, rj, r0m, r0j, k);
-        const double vj = vol[g];
+    for (int t0 = 0; t0 < nn; t0 += EB) {
+        int g[EB];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            F[d] = F[d] + (fn * (rj[d] - rm[d])) * vj;
+        for (int t = 0; t < EB; ++t) {
+            const int j = t0 + t < nn ? row[t0 + t] : -1;
+            g[t] = j > (int)m ? inv[j] : -1;
+        }
+        NodeRec o[EB];
+        double v[EB];
+#pragma unroll
+        for (int t = 0; t < EB; ++t) {
+            if (g[t] >= 0) {
+                o[t] = a.rec_old[g[t]];
+                v[t] = vol[g[t]];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < EB; ++t) {
+            if (g[t] < 0)
+                continue;
+            const double rj[3] = {o[t].b.x, o[t].b.y, o[t].b.z};
+            const double r0j[3] = {o[t].a.x, o[t].a.y, o[t].a.z};
+            const double fn = cython_fn(rm, rj, r0m, r0j, k);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                F[d] = F[d] + (fn * (rj[d] - rm[d])) * v[t];
+        }
     }
     const int fl = a.flags[p];
     const size_t b3 = 3 * (size_t)p;
