@@ -56,6 +56,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("FP64 bond-updates/s + %HBM peak at 1M nodes; batched vs plain "
           "sample-steps/s")
 HBM_PEAK_GBS = 8000.0
+REPEATS = 5  # timed regions of K steps; the median is reported
 B_NODE = {"euler": 156, "euler_cromer": 252, "velocity_verlet": 276}
 
 
@@ -125,7 +126,6 @@ def cpu_multicore(w, budget_s):
     n, k = w.nnodes, w.max_neighbours
     nlist = np.array(w.nlist, dtype=np.int32)
     u, ud, udd = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
-    r = np.zeros((n, 3))
     force, body = np.zeros((n, 3)), np.zeros((n, 3))
     d, f = w.magnitudes(10000)
     live = int(w.n_neigh.sum())
@@ -136,12 +136,11 @@ def cpu_multicore(w, budget_s):
     steps = 0
     t0 = time.perf_counter()
     while True:
-        L.oracle_positions(n, D(w.coords), D(u), D(r))
         L.oracle_slot_bond_force(
-            n, k, D(r), D(w.coords), I(nlist), D(w.volume),
+            n, k, D(u), D(w.coords), I(nlist), D(w.volume),
             float(w.bond_stiffness), float(w.critical_stretch), 0, None,
             None, None, None, None, None, None, 1, I(w.force_bc_types),
-            D(w.force_bc_values), f[steps], D(force), D(body))
+            D(w.force_bc_values), f[steps], D(force), D(body), 1)
         upd(n, D(force), D(u), D(ud), D(udd), I(w.bc_types),
             D(w.bc_values), D(w.densities), d[steps], w.damping, w.dt)
         steps += 1
@@ -154,8 +153,9 @@ def cpu_multicore(w, budget_s):
         "cores": cores, "kind": "port",
         "sample": (f"{steps} full steps of the {n}-node workload "
                    f"({elapsed:.1f} s): the per-node slot-layout bond force "
-                   "(cl/peridynamics.cl semantics, Cython stretch) and the "
-                   f"{w.integrator} update restated in C (gcc -O3, OpenMP)"),
+                   "and the "
+                   f"{w.integrator} update of cl/*.cl restated in C "
+                   "(gcc -O3, OpenMP)"),
     }
 
 
@@ -228,6 +228,36 @@ def composite_bench(steps, warmup=3):
         "algorithmic_bytes_per_launch": B,
         "hbm_frac": B / per / 1e9 / HBM_PEAK_GBS,
     }
+
+
+def exact_bench(steps, warmup=5):
+    """The Cython path's Euler on the 1,038,230-node beam (compact layout):
+    ms per step of the exact-order step (each node's bonds summed in
+    peridynamics.pyx's order, the default of integrators.Euler) and of the
+    fast-order step."""
+    from probabilistic_peridynamics_amd import configs
+    w = configs.beam_euler()
+    integ = configs.make_integrator(w)
+    dm, fm = w.magnitudes(warmup + 2 * (2 + REPEATS * steps))
+    out = {"workload": w.name, "nodes": w.nnodes,
+           "live_bonds": w.live_bonds(), "integrator": "euler (compact)"}
+    at = warmup
+    integ.run(warmup, dm[:at], fm[:at])
+    for mode in (True, False):
+        integ.set_exact_order(mode)
+        integ.run(2, dm[at:at + 2], fm[at:at + 2])
+        at += 2
+        reps = []
+        for r in range(REPEATS):
+            reps.append(integ.timed_run(steps, dm[at:at + steps],
+                                        fm[at:at + steps]) / steps)
+            at += steps
+        out["exact_ms_per_step" if mode else "fast_ms_per_step"] = float(
+            np.median(reps))
+    out["exact_over_fast"] = out["exact_ms_per_step"] / out["fast_ms_per_step"]
+    log(f"[bench] exact order: {out['exact_ms_per_step'] * 1e3:.1f} us/step, "
+        f"fast order {out['fast_ms_per_step'] * 1e3:.1f} us/step")
+    return out
 
 
 def adjoint_bench(steps, every, ndirs, dt):
@@ -360,6 +390,9 @@ def main():
     ap.add_argument("--uq-tune-samples", type=int, default=32)
     ap.add_argument("--composite-steps", type=int, default=50,
                     help="steps of the composite workload (0 to skip)")
+    ap.add_argument("--exact-steps", type=int, default=20,
+                    help="steps per timed region of the exact-order "
+                    "measurement (0 to skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint check (0 to skip)")
     ap.add_argument("--adjoint-dirs", type=int, default=32)
@@ -386,23 +419,33 @@ def main():
           f"K={w.max_neighbours} setup {setup_s:.1f}s tiles {tiles}",
           file=sys.stderr, flush=True)
 
-    dm, fm = w.magnitudes(args.warmup + args.steps)
+    # W untimed steps, then REPEATS timed regions of exactly K steps each
+    # (HIP events on the library's stream, synchronized on both sides); the
+    # median region is reported and all of them are listed
+    dm, fm = w.magnitudes(args.warmup + REPEATS * args.steps)
     integ.run(args.warmup, dm[:args.warmup], fm[:args.warmup])
     integ.synchronize()
     live0 = integ.live_bonds()
-    integ.synchronize()
+    reps = []
     t0 = time.perf_counter()
-    ms = integ.timed_run(args.steps, dm[args.warmup:], fm[args.warmup:])
-    integ.synchronize()
+    for r in range(REPEATS):
+        lo = args.warmup + r * args.steps
+        integ.synchronize()
+        reps.append(integ.timed_run(args.steps, dm[lo:lo + args.steps],
+                                    fm[lo:lo + args.steps]))
+        integ.synchronize()
     wall = time.perf_counter() - t0
     live1 = integ.live_bonds()
     live = 0.5 * (live0 + live1)
+    ms = float(np.median(reps))
     per_step_s = ms / 1e3 / args.steps
     value = live / per_step_s
     B = algorithmic_bytes(w, live)
     achieved = B / per_step_s / 1e9
-    print(f"[bench] {args.steps} steps: {ms:.3f} ms (events), {wall*1e3:.3f}"
-          f" ms wall, live {live0}->{live1}", file=sys.stderr, flush=True)
+    print(f"[bench] {REPEATS} x {args.steps} steps: "
+          + " ".join(f"{x:.3f}" for x in reps)
+          + f" ms (events, median {ms:.3f}), {wall*1e3:.1f} ms wall, live "
+          f"{live0}->{live1}", file=sys.stderr, flush=True)
 
     del integ
     uq = None
@@ -412,6 +455,10 @@ def main():
     comp = None
     if args.composite_steps > 0:
         comp = composite_bench(args.composite_steps)
+
+    exact = None
+    if args.exact_steps > 0:
+        exact = exact_bench(args.exact_steps)
 
     adj = None
     if args.adjoint_steps > 0:
@@ -426,6 +473,7 @@ def main():
             cpu["multicore"] = cpu_multicore(w, args.cpu_seconds)
 
     name, arch = _lib.device_name(0)
+    traffic = profile_traffic(w.name, args.lanes_per_node, args.tile_nodes)
     out = {
         "metric": METRIC,
         "value": value,
@@ -458,14 +506,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profile_traffic(w.name, args.lanes_per_node,
-                                       args.tile_nodes),
+            "traffic": traffic,
+            "traffic_frac": (None if traffic is None else
+                             traffic / per_step_s / 1e9 / HBM_PEAK_GBS),
             "algorithmic_bytes_per_launch": B,
             "launch_us": per_step_s * 1e6,
+            "repeats_ms_per_step": [x / args.steps for x in reps],
         },
         "cpu_baseline": cpu,
         "batched": uq,
         "composite": comp,
+        "exact_order": exact,
         "adjoint": adj,
     }
     print(json.dumps(out), flush=True)

@@ -114,6 +114,12 @@ int pd_create_buffers(pd_sim *sim, const int *nlist, const int *n_neigh,
  * re-running simulate() with new bond_stiffness (model.py:1523-1542). */
 int pd_set_node_stiffness(pd_sim *sim, const double *c);
 
+/* pd_set_node_stiffness with c (N) f64 in device memory of the simulation's
+ * device (for callers whose field lives on the GPU, e.g. a torch tensor).
+ * The caller's writes to c must be complete (its stream synchronized)
+ * before the call; c is copied, so it may be reused on return. */
+int pd_set_node_stiffness_device(pd_sim *sim, const double *c);
+
 /* Breaking criterion of the PMB model: 0 (default) breaks when
  * !(|s| < s0) (peridynamics.pyx:133); 1 breaks when !(s < s0)
  * (cl/peridynamics.cl:78). */
@@ -199,6 +205,12 @@ int pd_adjoint_start(pd_sim *sim, int nsteps, int checkpoint_every);
  * state as it was at the end of the recording. */
 int pd_adjoint_backward(pd_sim *sim, const double *dL_du, double *dL_dc);
 
+/* pd_adjoint_backward with dL_du and dL_dc in device memory of the
+ * simulation's device, caller's node order; nothing crosses PCIe.  dL_du
+ * must be complete before the call; dL_dc is complete on return. */
+int pd_adjoint_backward_device(pd_sim *sim, const double *dL_du,
+                               double *dL_dc);
+
 /* Free the recording. */
 int pd_adjoint_stop(pd_sim *sim);
 
@@ -225,6 +237,10 @@ int pd_monitor(pd_sim *sim, double *model, double *tips);
 int pd_write(pd_sim *sim, double *u, double *ud, double *udd, double *force,
              double *body_force, double *damage, int *nlist, int *n_neigh);
 
+/* u (N,3) and damage (N) into device memory of the simulation's device, in
+ * the caller's node order (either may be NULL); complete on return. */
+int pd_write_device(pd_sim *sim, double *u, double *damage);
+
 /* Bond regimes (N,K) i32 of the n-linear model (zeros for PMB). */
 int pd_get_regimes(pd_sim *sim, int *regimes);
 
@@ -236,15 +252,23 @@ int pd_synchronize(pd_sim *sim);
 
 void pd_destroy(pd_sim *sim);
 
+/* Flags of pd_bond_check. */
+#define PD_CHECK_TENSION 1 /* break on !(s < s0) instead of !(|s| < s0) */
+#define PD_CHECK_OPENCL 2  /* the slot layout's OpenCL stretch */
+
 /* Verification entry point: the step kernel's stretch pipeline on n bonds
  * (r_i, r_j, r0_i, r0_j each (n,3) f64), run on `device`.  Outputs per bond:
  * l = |r_j - r_i| and l0 = |r0_j - r0_i| (spatial.pyx:20-31, bitwise), the
  * kernel's working stretch s, the IEEE-division stretch (l - l0)/l0
  * (spatial.pyx:87-97) and the break decision !(|s| < s0)
- * (peridynamics.pyx:133; !(s < s0) when tension_only). */
+ * (peridynamics.pyx:133; !(s < s0) with PD_CHECK_TENSION).  With
+ * PD_CHECK_OPENCL, r_i and r_j are the displacements u_i, u_j and the
+ * lengths are cl/peridynamics.cl:65-75's, l = |(u_j - u_i) + xi| and
+ * l0 = |xi| with xi = r0_j - r0_i, their squared lengths contracted as
+ * fma(z, z, fma(x, x, y*y)). */
 int pd_bond_check(int device, int n, const double *ri, const double *rj,
                   const double *r0i, const double *r0j, double s0,
-                  int tension_only, double *l, double *l0, double *s,
+                  int flags, double *l, double *l0, double *s,
                   double *s_exact, int *broken);
 
 /* Neighbour list of a point cloud (no counterpart kernel in the reference:

@@ -16,7 +16,9 @@ per-step drivers:
 * :class:`SlotStepper` -- the per-slot ("-1" breaking) path of
   ``EulerCL/EulerCromerCL/VelocityVerletCL.__call__``
   (integrators.py:484-503, 586-606, 700-720) with the kernels of
-  peripy/cl/*.cl, the stretch computed with spatial.pyx's arithmetic.
+  peripy/cl/*.cl and their arithmetic as the OpenCL compiler that wrote the
+  reference's fixtures evaluated it (contracted squared lengths and updates);
+  ``arith=0`` gives the Cython stretch instead, for comparison.
 """
 import ctypes
 import pathlib
@@ -63,7 +65,12 @@ def lib():
             "oracle_slot_bond_force": (None, [
                 ctypes.c_int, ctypes.c_int, _D, _D, _I, _D, ctypes.c_double,
                 ctypes.c_double, ctypes.c_int, _D, _D, _I, _I, _D, _D, _D,
-                ctypes.c_int, _I, _D, ctypes.c_double, _D, _D]),
+                ctypes.c_int, _I, _D, ctypes.c_double, _D, _D,
+                ctypes.c_int]),
+            "oracle_cl_strain": (ctypes.c_double, [_D, _D, _D, _D, _D, _D]),
+            "oracle_update_euler_cl": (None, [
+                ctypes.c_int, _D, _D, _I, _D, ctypes.c_double,
+                ctypes.c_double]),
             "oracle_slot_damage": (None, [
                 ctypes.c_int, ctypes.c_int, _I, _I, _I, _D]),
             "oracle_update_euler_cromer": (None, [
@@ -137,6 +144,16 @@ def strain2(l, r10, r20):
     """spatial.pyx:69-97."""
     a = [np.ascontiguousarray(x, dtype=np.float64) for x in (r10, r20)]
     return lib().oracle_strain2(float(l), *[_d(x) for x in a])
+
+
+def cl_strain(ui, uj, r0i, r0j):
+    """cl/peridynamics.cl:65-75 as contracted: (s, l, l0) of one bond from
+    the displacements and initial positions of its ends."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ui, uj, r0i, r0j)]
+    l, l0 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    s = lib().oracle_cl_strain(*[_d(x) for x in a], ctypes.byref(l),
+                               ctypes.byref(l0))
+    return s, l.value, l0.value
 
 
 def damage(n_neigh, family):
@@ -278,9 +295,12 @@ EULER, EULER_CROMER, VELOCITY_VERLET = 0, 1, 2
 class SlotStepper:
     """Per-slot ("-1") step: bond_force1..4 semantics + an integrator update.
 
-    ``integrator``: 0 Euler (cl/euler.cl, identical to peridynamics.pyx's
-    update), 1 Euler-Cromer (cl/euler_cromer.cl), 2 Velocity-Verlet
-    (cl/velocity_verlet.cl).  Damage model: scalar ``bond_stiffness`` /
+    ``integrator``: 0 Euler (cl/euler.cl), 1 Euler-Cromer
+    (cl/euler_cromer.cl), 2 Velocity-Verlet (cl/velocity_verlet.cl).
+    ``arith``: 1 (default) the OpenCL kernels' arithmetic as compiled for the
+    reference's fixtures -- xi = r0_j - r0_i, xi_eta = (u_j - u_i) + xi,
+    squared lengths and updates contracted into fused multiply-adds; 0 the
+    Cython stretch (r = r0 + u, spatial.pyx) and uncontracted updates.  Damage model: scalar ``bond_stiffness`` /
     ``critical_stretch`` (PMB), optionally a per-node stiffness field
     ``cnode`` (k_ij = (c_i + c_j)/2), optional ``stiffness_corrections``;
     or (nbond_types, nregimes) tables with ``bond_types`` and ``regimes``.
@@ -292,7 +312,7 @@ class SlotStepper:
                  bond_stiffness, critical_stretch, damping=0.0,
                  densities=None, stiffness_corrections=None, bond_types=None,
                  regimes=None, plus_cs=None, nregimes=1, cnode=None,
-                 tension_only=False, u=None, ud=None, udd=None):
+                 tension_only=False, u=None, ud=None, udd=None, arith=1):
         f64 = lambda a: None if a is None else np.array(a, dtype=np.float64)
         i32 = lambda a: None if a is None else np.array(a, dtype=np.int32)
         self.integrator = int(integrator)
@@ -311,6 +331,7 @@ class SlotStepper:
         self.corr = f64(stiffness_corrections)
         self.cnode = f64(cnode)
         self.tension_only = int(bool(tension_only))
+        self.arith = int(arith)
         self.nregimes = int(nregimes)
         if np.ndim(bond_stiffness) == 0:
             self.k = float(bond_stiffness)
@@ -335,20 +356,23 @@ class SlotStepper:
     def __call__(self, displacement_bc_magnitude, force_bc_magnitude):
         L = lib()
         n, k = self.nlist.shape
-        r = np.empty_like(self.u)
-        L.oracle_positions(n, _d(self.coords), _d(self.u), _d(r))
         L.oracle_slot_bond_force(
-            n, k, _d(r), _d(self.coords), _i(self.nlist), _d(self.volume),
-            self.k, self.s0, self.tension_only, _d(self.cnode),
-            _d(self.corr), _i(self.bond_types), _i(self.regimes),
-            _d(self.k_tab), _d(self.s_tab), _d(self.pc_tab), self.nregimes,
-            _i(self.force_bc_types), _d(self.force_bc_values),
-            float(force_bc_magnitude), _d(self.force), _d(self.body_force))
+            n, k, _d(self.u), _d(self.coords), _i(self.nlist),
+            _d(self.volume), self.k, self.s0, self.tension_only,
+            _d(self.cnode), _d(self.corr), _i(self.bond_types),
+            _i(self.regimes), _d(self.k_tab), _d(self.s_tab),
+            _d(self.pc_tab), self.nregimes, _i(self.force_bc_types),
+            _d(self.force_bc_values), float(force_bc_magnitude),
+            _d(self.force), _d(self.body_force), self.arith)
         if self.integrator == EULER:
-            L.oracle_update_displacement(
-                n, _d(self.u), _d(self.bc_values), _i(self.bc_types),
-                _d(self.force), float(displacement_bc_magnitude), self.dt)
+            fn = (L.oracle_update_euler_cl if self.arith
+                  else L.oracle_update_displacement)
+            fn(n, _d(self.u), _d(self.bc_values), _i(self.bc_types),
+               _d(self.force), float(displacement_bc_magnitude), self.dt)
         else:
+            if not self.arith:
+                raise ValueError("the dynamic integrators are restated with "
+                                 "the OpenCL arithmetic only (arith=1)")
             fn = (L.oracle_update_euler_cromer
                   if self.integrator == EULER_CROMER
                   else L.oracle_update_velocity_verlet)

@@ -20,9 +20,15 @@
  *  (2) The semantics the reference only has in OpenCL (per-slot "-1" bond
  *      breaking, stiffness corrections, the n-linear / composite regime state
  *      machine, the Euler-Cromer and Velocity-Verlet updates), restated from
- *      the peripy/cl kernels, but with the stretch computed by the Cython arithmetic
- *      of (1) (r = r0 + u per node, then spatial.pyx's ceuclid/cstrain).  This
- *      is the definition the HIP library implements for those integrators.
+ *      the peripy/cl kernels with their own arithmetic: xi = r0_j - r0_i,
+ *      xi_eta = (u_j - u_i) + xi, the squared lengths contracted into fused
+ *      multiply-adds as the OpenCL compiler that wrote the reference's
+ *      Euler-Cromer and Velocity-Verlet fixtures evaluated them (those
+ *      fixtures are met to ~6e-19 in u with this arithmetic, ~2e-16 without
+ *      the contraction and ~7e-15 with the Cython stretch), and the
+ *      integrator updates contracted likewise.  An `arith` switch keeps the
+ *      Cython stretch available for comparison.  This is the definition the
+ *      HIP library implements for the slot-layout integrators.
  *
  * Pinning (see tests/test_oracle_*.py): the reference's own known-answer
  * tests (test_peridynamics.py, test_spatial.py, test_cl_euler*.py,
@@ -233,44 +239,95 @@ void oracle_create_crack(int ncrack, const int *crack, int max_neigh,
 /* (2) OpenCL-only semantics restated with the Cython stretch arithmetic  */
 /* ---------------------------------------------------------------------- */
 
-/* Per-slot bond force with in-place breaking, the structure of
- * cl/peridynamics.cl bond_force1..4 (:4-131, :134-262, :265-414, :417-565):
- * every node i visits every slot p of its row; nlist[i][p] == -1 is a broken
- * (or padding) slot.  Differences from the OpenCL text, by design:
- *   - the stretch is spatial.pyx's: l = |r_j - r_i| with r = r0 + u formed per
- *     node by the caller, l0 = |r0_j - r0_i|, s = (l - l0)/l0;
- *   - the force is written in peridynamics.pyx:77-89's order:
- *     fn = s*k [*corr]; fn = fn/l; f_d = fn*(r_j,d - r_i,d); F_i,d += f_d*V_j
- *     (n-linear: fn = s*k[t,r] + plus_cs[t,r] [*corr] before the division);
- *   - PMB breaking uses peridynamics.pyx:133's !(|s| < s0) unless
- *     tension_only, which gives cl/peridynamics.cl:78's !(s < s0);
- *   - with a per-node stiffness field cnode, k_ij = (c_i + c_j)*0.5.
- * The n-linear state machine is cl/peridynamics.cl:326-367 verbatim in
- * meaning (at most one regime change per step, break when regime >= nregimes),
- * except that the table is only read for a live slot.
- * Tables are (nbond_types, nregimes) flattened t*nregimes + r
- * (model.py:1042-1070).  body_force = bond sum; force = bond sum + BC term
- * (cl/peridynamics.cl:117-129). */
+/* Squared length of (x, y, z) as cl/peridynamics.cl:72-73 evaluates
+ * x*x + y*y + z*z once contracted: fma(z, z, fma(x, x, y*y)). */
+static double cl_sq(double x, double y, double z)
+{
+    return fma(z, z, fma(x, x, y * y));
+}
+
+/* The stretch of one bond with cl/peridynamics.cl:65-75's arithmetic as
+ * contracted: xi = r0_j - r0_i, xe = (u_j - u_i) + xi, l0 = sqrt(|xi|^2),
+ * l = sqrt(|xe|^2); returns (l - l0)/l0 and stores l, l0. */
+double oracle_cl_strain(const double *ui, const double *uj, const double *r0i,
+                        const double *r0j, double *l_out, double *l0_out)
+{
+    double xi[3], xe[3];
+    for (int d = 0; d < 3; ++d) {
+        xi[d] = r0j[d] - r0i[d];
+        xe[d] = (uj[d] - ui[d]) + xi[d];
+    }
+    double l0 = sqrt(cl_sq(xi[0], xi[1], xi[2]));
+    double l = sqrt(cl_sq(xe[0], xe[1], xe[2]));
+    *l_out = l;
+    *l0_out = l0;
+    return (l - l0) / l0;
+}
+
+/* Per-slot bond force with in-place breaking, cl/peridynamics.cl
+ * bond_force1..4 (:4-131, :134-262, :265-414, :417-565): every node i visits
+ * every slot p of its row; nlist[i][p] == -1 is a broken (or padding) slot.
+ *   arith 1 (the OpenCL kernels, :65-75 and :206-216):
+ *     xi = r0_j - r0_i; xe = (u_j - u_i) + xi; l0 = sqrt(|xi|^2),
+ *     y = sqrt(|xe|^2) (squared lengths contracted, cl_sq); s = (y - l0)/l0;
+ *     f = s*k*V_j (bond_force1), s*k*corr*V_j (bond_force2),
+ *     fma(s, k[t,r], plus_cs[t,r])*V_j (bond_force3; times corr before V_j
+ *     in bond_force4); F_i,d += f*(xe_d/y).
+ *   arith 0 (the Cython stretch, spatial.pyx:20-31, :87-97): r = r0 + u per
+ *     node, l = |r_j - r_i|, l0 = |r0_j - r0_i|, s = (l - l0)/l0, and the
+ *     force in peridynamics.pyx:77-89's order: fn = s*k [*corr]; fn = fn/l;
+ *     F_i,d += (fn*(r_j,d - r_i,d))*V_j.
+ * PMB breaking is peridynamics.pyx:133's !(|s| < s0) unless tension_only,
+ * which gives cl/peridynamics.cl:78's !(s < s0).  With a per-node
+ * stiffness field cnode, k_ij = (c_i + c_j)*0.5.  The n-linear state machine
+ * is cl/peridynamics.cl:326-367 (at most one regime change per step, break
+ * when regime >= nregimes), except that the table is only read for a live
+ * slot.  Tables are (nbond_types, nregimes) flattened t*nregimes + r
+ * (model.py:1042-1070).  Slots are summed in row order from 0.0 (the OpenCL
+ * kernel sums them by a tree).  body_force = bond sum; force = bond sum + BC
+ * term (cl/peridynamics.cl:117-129; contracted with arith 1). */
 void oracle_slot_bond_force(
-    int nnodes, int max_neigh, const double *r, const double *r0,
+    int nnodes, int max_neigh, const double *u, const double *r0,
     int *nlist, const double *volume, double bond_stiffness,
     double critical_stretch, int tension_only, const double *cnode,
     const double *corr, const int *bond_types, int *regimes,
     const double *k_tab, const double *s_tab, const double *pc_tab,
     int nregimes, const int *force_bc_types, const double *force_bc_values,
-    double force_bc_scale, double *force, double *body_force)
+    double force_bc_scale, double *force, double *body_force, int arith)
 {
 #pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (int i = 0; i < nnodes; ++i) {
         double acc[3] = {0.0, 0.0, 0.0};
+        const double *r0i = &r0[3 * (size_t)i], *ui = &u[3 * (size_t)i];
+        double ri[3];
+        for (int d = 0; d < 3; ++d)
+            ri[d] = r0i[d] + ui[d];
         for (int p = 0; p < max_neigh; ++p) {
             size_t slot = (size_t)i * max_neigh + p;
             int j = nlist[slot];
             if (j == -1)
                 continue;
-            double l = oracle_euclid(&r[3 * (size_t)i], &r[3 * (size_t)j]);
-            double s = oracle_strain2(l, &r0[3 * (size_t)i], &r0[3 * (size_t)j]);
-            double fn;
+            const double *r0j = &r0[3 * (size_t)j], *uj = &u[3 * (size_t)j];
+            double dir[3], l, s;
+            if (arith) {
+                double xi[3];
+                for (int d = 0; d < 3; ++d) {
+                    xi[d] = r0j[d] - r0i[d];
+                    dir[d] = (uj[d] - ui[d]) + xi[d];
+                }
+                double l0 = sqrt(cl_sq(xi[0], xi[1], xi[2]));
+                l = sqrt(cl_sq(dir[0], dir[1], dir[2]));
+                s = (l - l0) / l0;
+            } else {
+                double rj[3];
+                for (int d = 0; d < 3; ++d) {
+                    rj[d] = r0j[d] + uj[d];
+                    dir[d] = rj[d] - ri[d];
+                }
+                l = oracle_euclid(ri, rj);
+                s = oracle_strain2(l, r0i, r0j);
+            }
+            double k;
             if (k_tab == NULL) {
                 int broken = tension_only ? !(s < critical_stretch)
                                           : !(fabs(s) < critical_stretch);
@@ -278,10 +335,9 @@ void oracle_slot_bond_force(
                     nlist[slot] = -1;
                     continue;
                 }
-                double k = bond_stiffness;
+                k = bond_stiffness;
                 if (cnode != NULL)
                     k = (cnode[i] + cnode[j]) * 0.5;
-                fn = s * k;
             } else {
                 int t = bond_types[slot];
                 int reg = regimes[slot];
@@ -298,22 +354,39 @@ void oracle_slot_bond_force(
                     nlist[slot] = -1;
                     continue;
                 }
-                fn = s * k_tab[t * nregimes + reg] + pc_tab[t * nregimes + reg];
+                k = k_tab[t * nregimes + reg];
             }
-            if (corr != NULL)
-                fn = fn * corr[slot];
-            fn = fn / l;
-            for (int d = 0; d < 3; ++d) {
-                double f = fn * (r[3 * (size_t)j + d] - r[3 * (size_t)i + d]);
-                acc[d] = acc[d] + f * volume[j];
+            if (arith) {
+                double f = k_tab == NULL
+                               ? s * k
+                               : fma(s, k, pc_tab[bond_types[slot] * nregimes +
+                                                  regimes[slot]]);
+                if (corr != NULL)
+                    f = f * corr[slot];
+                f = f * volume[j];
+                for (int d = 0; d < 3; ++d)
+                    acc[d] = acc[d] + f * (dir[d] / l);
+            } else {
+                double fn = k_tab == NULL
+                                ? s * k
+                                : s * k + pc_tab[bond_types[slot] * nregimes +
+                                                 regimes[slot]];
+                if (corr != NULL)
+                    fn = fn * corr[slot];
+                fn = fn / l;
+                for (int d = 0; d < 3; ++d)
+                    acc[d] = acc[d] + (fn * dir[d]) * volume[j];
             }
         }
         for (int d = 0; d < 3; ++d) {
             size_t q = 3 * (size_t)i + d;
             body_force[q] = acc[d];
-            force[q] = force_bc_types[q] == 0
-                           ? acc[d]
-                           : acc[d] + force_bc_scale * force_bc_values[q];
+            if (force_bc_types[q] == 0)
+                force[q] = acc[d];
+            else if (arith)
+                force[q] = fma(force_bc_scale, force_bc_values[q], acc[d]);
+            else
+                force[q] = acc[d] + force_bc_scale * force_bc_values[q];
         }
     }
 }
@@ -334,8 +407,20 @@ void oracle_slot_damage(int nnodes, int max_neigh, const int *nlist,
     }
 }
 
-/* cl/euler_cromer.cl:29-34, per DOF q:
- * udd = (F - eta*ud)/rho; ud += udd*dt; u = bc ? scale*val : u + dt*ud */
+/* cl/euler.cl:21-23 as contracted: u = bc ? scale*val : fma(dt, F, u) */
+void oracle_update_euler_cl(int nnodes, double *u, const double *bc_values,
+                            const int *bc_types, const double *force,
+                            double bc_scale, double dt)
+{
+#pragma omp parallel for schedule(static) if (nnodes > 4096)
+    for (size_t q = 0; q < 3 * (size_t)nnodes; ++q)
+        u[q] = bc_types[q] == 0 ? fma(dt, force[q], u[q])
+                                : bc_scale * bc_values[q];
+}
+
+/* cl/euler_cromer.cl:29-34 as contracted, per DOF q:
+ * udd = fma(-eta, ud, F)/rho; ud = fma(udd, dt, ud);
+ * u = bc ? scale*val : fma(dt, ud, u) */
 void oracle_update_euler_cromer(int nnodes, const double *force, double *u,
                                 double *ud, double *udd, const int *bc_types,
                                 const double *bc_values,
@@ -344,16 +429,18 @@ void oracle_update_euler_cromer(int nnodes, const double *force, double *u,
 {
 #pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q) {
-        double uddi = (force[q] - damping * ud[q]) / densities[q];
+        double uddi = fma(-damping, ud[q], force[q]) / densities[q];
         udd[q] = uddi;
-        ud[q] = ud[q] + uddi * dt;
-        u[q] = bc_types[q] == 0 ? (u[q] + dt * ud[q]) : (bc_scale * bc_values[q]);
+        ud[q] = fma(uddi, dt, ud[q]);
+        u[q] = bc_types[q] == 0 ? fma(dt, ud[q], u[q])
+                                : (bc_scale * bc_values[q]);
     }
 }
 
-/* cl/velocity_verlet.cl:29-35, per DOF q:
- * ud1 = ud + (dt/2)*udd; udd1 = (F - eta*ud1)/rho; ud = ud1 + (dt/2)*udd1;
- * udd = udd1; u = bc ? scale*val : u + dt*(ud + (dt/2)*udd1) */
+/* cl/velocity_verlet.cl:29-35 as contracted, per DOF q:
+ * ud1 = fma(dt/2, udd, ud); udd1 = fma(-eta, ud1, F)/rho;
+ * ud = fma(dt/2, udd1, ud1); udd = udd1;
+ * u = bc ? scale*val : fma(dt, fma(dt/2, udd1, ud), u) */
 void oracle_update_velocity_verlet(int nnodes, const double *force, double *u,
                                    double *ud, double *udd,
                                    const int *bc_types, const double *bc_values,
@@ -362,11 +449,11 @@ void oracle_update_velocity_verlet(int nnodes, const double *force, double *u,
 {
 #pragma omp parallel for schedule(static) if (nnodes > 4096)
     for (size_t q = 0; q < 3 * (size_t)nnodes; ++q) {
-        double ud1 = ud[q] + (dt / 2) * udd[q];
-        double udd1 = (force[q] - damping * ud1) / densities[q];
-        ud[q] = ud1 + (dt / 2) * udd1;
+        double ud1 = fma(dt / 2, udd[q], ud[q]);
+        double udd1 = fma(-damping, ud1, force[q]) / densities[q];
+        ud[q] = fma(dt / 2, udd1, ud1);
         udd[q] = udd1;
-        u[q] = bc_types[q] == 0 ? (u[q] + dt * (ud[q] + (dt / 2) * udd1))
+        u[q] = bc_types[q] == 0 ? fma(dt, fma(dt / 2, udd1, ud[q]), u[q])
                                 : (bc_scale * bc_values[q]);
     }
 }

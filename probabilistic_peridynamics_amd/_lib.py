@@ -45,6 +45,7 @@ SIGNATURES = {
         _SIM, _I, _I, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D, _D,
         _I]),
     "pd_set_node_stiffness": (ctypes.c_int, [_SIM, _D]),
+    "pd_set_node_stiffness_device": (ctypes.c_int, [_SIM, ctypes.c_void_p]),
     "pd_set_break_mode": (ctypes.c_int, [_SIM, ctypes.c_int]),
     "pd_set_exact_order": (ctypes.c_int, [_SIM, ctypes.c_int]),
     "pd_set_launch": (ctypes.c_int, [_SIM, ctypes.c_int, ctypes.c_int]),
@@ -53,10 +54,14 @@ SIGNATURES = {
     "pd_run_graph": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D, _F]),
     "pd_adjoint_start": (ctypes.c_int, [_SIM, ctypes.c_int, ctypes.c_int]),
     "pd_adjoint_backward": (ctypes.c_int, [_SIM, _D, _D]),
+    "pd_adjoint_backward_device": (ctypes.c_int, [_SIM, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "pd_adjoint_stop": (ctypes.c_int, [_SIM]),
     "pd_run": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D]),
     "pd_run_timed": (ctypes.c_int, [_SIM, ctypes.c_int, _D, _D, _F]),
     "pd_write": (ctypes.c_int, [_SIM, _D, _D, _D, _D, _D, _D, _I, _I]),
+    "pd_write_device": (ctypes.c_int, [_SIM, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
     "pd_set_tips": (ctypes.c_int, [_SIM, ctypes.c_int, _I, _I, _I,
                                    ctypes.c_int]),
     "pd_monitor": (ctypes.c_int, [_SIM, _D, _D]),
@@ -235,16 +240,19 @@ def neighbour_list(coords, horizon, max_neighbours=None, pad_value=0,
     return family, nlist
 
 
-def bond_check(ri, rj, r0i, r0j, s0, tension_only=False, device=0):
+def bond_check(ri, rj, r0i, r0j, s0, tension_only=False, opencl=False,
+               device=0):
     """The step kernel's stretch pipeline on a batch of bonds (GPU).
 
-    Returns dict(l, l0, s, s_exact, broken)."""
+    ``opencl``: ri, rj are displacements and the stretch is the slot
+    layout's OpenCL one.  Returns dict(l, l0, s, s_exact, broken)."""
     ri, rj, r0i, r0j = (as_f64(a) for a in (ri, rj, r0i, r0j))
     n = ri.shape[0]
     out = {k: np.empty(n) for k in ("l", "l0", "s", "s_exact")}
     out["broken"] = np.empty(n, dtype=np.int32)
     check(lib().pd_bond_check(
         device, n, dptr(ri), dptr(rj), dptr(r0i), dptr(r0j), float(s0),
-        int(bool(tension_only)), dptr(out["l"]), dptr(out["l0"]),
+        int(bool(tension_only)) | (2 if opencl else 0), dptr(out["l"]),
+        dptr(out["l0"]),
         dptr(out["s"]), dptr(out["s_exact"]), iptr(out["broken"])))
     return out

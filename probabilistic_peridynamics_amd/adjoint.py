@@ -7,7 +7,9 @@ run on the GPU.  The backward pass is the library's hand-written adjoint
 broke and checkpoints ``u``; the backward pass replays each checkpoint
 segment with the step kernel and runs the adjoint kernel back through it,
 holding the recorded break mask fixed.  :class:`StiffnessField` wraps this
-as a ``torch.autograd.Function`` of the log-stiffness ``g``.
+as a ``torch.autograd.Function`` of the log-stiffness ``g``; a ``g`` on the
+GPU stays there (the field, u and the gradient pass to and from the library
+by device address), a ``g`` on the host goes through numpy.
 """
 import numpy as np
 
@@ -35,27 +37,70 @@ class StiffnessAdjoint:
         self.integrator.set_launch(lanes_per_node, tile_nodes)
         self.dmags, self.fmags = w.magnitudes(self.steps)
         self.c = None
+        # incremented by every forward: the recording a backward uses is
+        # that of the last forward
+        self.generation = 0
 
-    def forward(self, g):
-        """u after ``steps`` steps from rest, for c = c0 exp(g)."""
+    def _restart(self):
         w = self.w
-        self.c = self.c0 * np.exp(np.asarray(g, dtype=np.float64))
-        integ = self.integrator
-        integ.set_node_stiffness(self.c)
         z = np.zeros((w.nnodes, 3))
-        integ.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,
-                             w.critical_stretch, None, z, z, z, z, z,
-                             np.zeros(w.nnodes), None, 1, 1)
+        self.integrator.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,
+                                       w.critical_stretch, None, z, z, z, z,
+                                       z, np.zeros(w.nnodes), None, 1, 1)
+
+    def _record(self):
+        integ = self.integrator
         integ.adjoint_start(self.steps, self.every)
         integ.run(self.steps, self.dmags, self.fmags)
-        return integ.write(None, None, None, None, None, None, None,
-                           None)[0]
+        self.generation += 1
+
+    def forward(self, g):
+        """u after ``steps`` steps from rest, for c = c0 exp(g) (numpy)."""
+        self.c = self.c0 * np.exp(np.asarray(g, dtype=np.float64))
+        self.integrator.set_node_stiffness(self.c)
+        self._restart()
+        self._record()
+        return self.integrator.write(None, None, None, None, None, None,
+                                     None, None)[0]
 
     def backward(self, dL_du):
-        """dL/dg for dL/du of the last :meth:`forward`."""
+        """dL/dg for dL/du of the last :meth:`forward` (numpy)."""
         if self.c is None:
             raise RuntimeError("forward() has not been called")
         return self.c * self.integrator.adjoint_backward(dL_du)
+
+    def forward_device(self, g):
+        """:meth:`forward` for a float64 torch tensor g on this solver's
+        GPU: c = c0 exp(g) is formed there and handed to the library by
+        address, and u comes back as a tensor on the same GPU."""
+        import torch
+        g = g.detach().contiguous()
+        c = self.c0 * torch.exp(g)
+        torch.cuda.current_stream(g.device).synchronize()
+        self._restart()
+        self.integrator.set_node_stiffness_device(c.data_ptr())
+        self.c = c
+        self._record()
+        u = torch.empty((self.w.nnodes, 3), dtype=torch.float64,
+                        device=g.device)
+        self.integrator.write_device(u.data_ptr(), None)
+        return u
+
+    def backward_device(self, dL_du):
+        """:meth:`backward` for a float64 torch tensor dL/du (nnodes, 3) on
+        this solver's GPU; dL/dg is returned on the same GPU."""
+        import torch
+        if self.c is None:
+            raise RuntimeError("forward() has not been called")
+        dL_du = dL_du.detach().contiguous()
+        torch.cuda.current_stream(dL_du.device).synchronize()
+        dc = torch.empty(self.w.nnodes, dtype=torch.float64,
+                         device=dL_du.device)
+        self.integrator.adjoint_backward_device(dL_du.data_ptr(),
+                                                dc.data_ptr())
+        c = self.c if torch.is_tensor(self.c) else torch.from_numpy(
+            self.c).to(dL_du.device)
+        return c * dc
 
     def state(self):
         """(u, damage, nlist) after the last forward."""
@@ -68,19 +113,42 @@ def _function():
     import torch
 
     class StiffnessField(torch.autograd.Function):
-        """u_N(g) with the adjoint as its backward."""
+        """u_N(g) with the adjoint as its backward.
+
+        The solver holds one recording, that of its last forward.  A node
+        of the graph whose recording was replaced by a later forward (two
+        evaluations before one backward) re-runs its own forward first, so
+        its gradient is always that of its own g."""
 
         @staticmethod
         def forward(ctx, g, solver):
-            u = solver.forward(g.detach().cpu().numpy())
+            if g.dtype != torch.float64:
+                raise TypeError("g must be float64")
+            if g.is_cuda:
+                u = solver.forward_device(g)
+            else:
+                u = torch.from_numpy(solver.forward(g.detach().numpy()))
             ctx.solver = solver
-            return torch.from_numpy(u).to(device=g.device, dtype=g.dtype)
+            ctx.generation = solver.generation
+            ctx.save_for_backward(g)
+            return u
 
         @staticmethod
         def backward(ctx, grad_u):
-            dg = ctx.solver.backward(grad_u.detach().cpu().numpy())
-            return torch.from_numpy(dg).to(device=grad_u.device,
-                                           dtype=grad_u.dtype), None
+            solver = ctx.solver
+            (g,) = ctx.saved_tensors
+            if solver.generation != ctx.generation:
+                # the recording belongs to another g: record this one again
+                if g.is_cuda:
+                    solver.forward_device(g)
+                else:
+                    solver.forward(g.detach().numpy())
+            if grad_u.is_cuda:
+                dg = solver.backward_device(grad_u.to(torch.float64))
+            else:
+                dg = torch.from_numpy(solver.backward(
+                    grad_u.detach().to(torch.float64).numpy()))
+            return dg.to(dtype=g.dtype), None
 
     return StiffnessField
 

@@ -81,6 +81,7 @@ class Workload:
     force_bc_values: np.ndarray = None
     bond_types: np.ndarray = None
     plus_cs: np.ndarray = None
+    stiffness_corrections: np.ndarray = None  # (N, K) factors, slot layout
     nregimes: int = 1
     nbond_types: int = 1
     disp_rate: float = 0.0             # displacement BC magnitude per step
@@ -209,6 +210,20 @@ def beam_vv(shape=(470, 47, 47), seed=1, layout="slot"):
         extra={"dx": dx, "shape": shape, "seed": seed})
 
 
+def beam_euler(shape=(470, 47, 47), seed=1):
+    """The 1,038,230-node beam of beam_vv stepped by the Cython path's
+    Euler integrator (compact layout): same mesh, supports and body-force
+    strip, no densities, a step (5e-15 s) inside the overdamped stability
+    limit."""
+    w = beam_vv(shape, seed, layout="compact")
+    w.name = f"beam_euler_{w.nnodes}"
+    w.integrator = "euler"
+    w.densities = None
+    w.dt = 5e-15
+    w.extra["layout"] = "compact"
+    return w
+
+
 def trilinear_tables(c_a):
     """(nbond_types=3, nregimes=3) tables of the composite trilinear law and
     their plus_cs (model.py:1050-1070's recurrence)."""
@@ -258,7 +273,9 @@ def make_integrator(w, context=None, lanes_per_node=None, tile_nodes=None):
     """A built integrator with the workload's state uploaded (launch shape
     optional: see Integrator.set_launch)."""
     from . import integrators as it
-    if w.integrator == "euler":
+    if w.integrator == "euler" and w.extra.get("layout") == "compact":
+        integ = it.Euler(w.dt)
+    elif w.integrator == "euler":
         integ = it.EulerHIP(w.dt, context=context)
     elif w.integrator == "euler_cromer":
         integ = it.EulerCromerHIP(w.damping, w.dt, context=context)
@@ -266,7 +283,7 @@ def make_integrator(w, context=None, lanes_per_node=None, tile_nodes=None):
         integ = it.VelocityVerletHIP(w.damping, w.dt, context=context)
     integ.build(w.nnodes, 3, w.max_neighbours, w.coords, w.volume, w.family,
                 w.bc_types, w.bc_values, w.force_bc_types, w.force_bc_values,
-                None, w.bond_types, w.densities)
+                w.stiffness_corrections, w.bond_types, w.densities)
     if lanes_per_node is not None or tile_nodes is not None:
         integ.set_launch(lanes_per_node or 4, tile_nodes or 128)
     n = w.nnodes

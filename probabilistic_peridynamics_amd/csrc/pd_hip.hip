@@ -30,7 +30,8 @@
 // runs, so per-node loads and stores coalesce.  Inputs are permuted into
 // this order on upload and outputs back on download.
 //   rec[2][N]  NodeRec, 64 bytes: (r0x, r0y, r0z, w | rx, ry, rz, v) with
-//              r = r0 + u and the bond loop's weights w (4kV for the scalar
+//              r = r0 + u (compact layout) or r = u (slot layout, whose
+//              stretch is formed from u), the bond loop's weights w (4kV for the scalar
 //              model, 2V for a stiffness field c, V for n-linear models) and
 //              v (2cV for a field, else 0); ping-pong, with r0, w and v
 //              static in both.
@@ -61,16 +62,22 @@
 //              (Cython) layout, in the caller's numbering, maintained by
 //              in-kernel compaction.
 //
-// Stretch arithmetic: exactly spatial.pyx's (dr = r_j - r_i,
-// l = sqrt((dr0^2 + dr1^2) + dr2^2), s = (l - l0)/l0), built without
-// floating-point contraction, so l and l0 are bitwise those of the Cython
-// code.  The square roots use the correctly rounded rsq-Goldschmidt sequence
-// that the compiler emits for sqrt(double), without its range scaling (taken
-// only for q outside [2^-767, inf), where the library sqrt is called).  Its
-// by-product h ~ 0.5/sqrt(q) (relative error ~1e-13) replaces the two
-// divisions; a stretch within a relative 1e-10 of any threshold it is
-// compared with is recomputed with the IEEE division, so every break and
-// regime decision equals the one made with the correctly rounded s.
+// Stretch arithmetic.  Compact layout (the Cython path): exactly
+// spatial.pyx's (dr = r_j - r_i with r = r0 + u, l = sqrt((dr0^2 + dr1^2) +
+// dr2^2), s = (l - l0)/l0), built without floating-point contraction, so l
+// and l0 are bitwise those of the Cython code.  Slot layout (the OpenCL
+// integrators): cl/peridynamics.cl:65-75's (xi = r0_j - r0_i, xi_eta =
+// (u_j - u_i) + xi), the squared lengths contracted as
+// fma(z, z, fma(x, x, y*y)) and the updates contracted, as the OpenCL
+// compiler that wrote the reference's Euler-Cromer and Velocity-Verlet
+// fixtures evaluated them.  The square roots use the correctly rounded
+// rsq-Goldschmidt sequence that the compiler emits for sqrt(double), without
+// its range scaling (taken only for q outside [2^-767, inf), where the
+// library sqrt is called).  Its by-product h ~ 0.5/sqrt(q) (relative error
+// ~1e-13) replaces the two divisions; a stretch within a relative 1e-10 of
+// any threshold it is compared with is recomputed with the IEEE division, so
+// every break and regime decision equals the one made with the correctly
+// rounded s.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -173,7 +180,12 @@ struct StepArgs {
     const double *cnode;  // node stiffness c (DM_FIELD)
     int *brk;  // if set: the step at which each slot's bond broke
     int step;  // index of this step (for brk)
-    int exact;  // k_exact_euler does the force and update of this step
+    // exact-order step: lower-slot counts, upper-list lengths and the upper
+    // lists (slot index << 16 | halo loc), the scalar stiffness
+    const int *nlow;
+    int *nup;
+    unsigned *uord;
+    double kx;
 };
 
 __host__ __device__ __forceinline__ size_t slot_of(long long chunk0,
@@ -299,14 +311,27 @@ constexpr unsigned piece_bytes()
     return 16u * (DM == DM_NLIN ? LDS_CAP_NLIN : LDS_CAP_PMB);
 }
 
+// Squared length of (x, y, z): spatial.pyx's ((x*x + y*y) + z*z), uncontracted
+// (CLA false), or cl/peridynamics.cl:72-73's as contracted,
+// fma(z, z, fma(x, x, y*y)) (CLA true).
+template <bool CLA>
+__device__ __forceinline__ double sq3(double x, double y, double z)
+{
+    if (CLA)
+        return __builtin_fma(z, z, __builtin_fma(x, x, y * y));
+    return (x * x + y * y) + z * z;
+}
+
 // W bonds of node i at slots s0 .. s0+W-1: ev their halo indices (LOC_DEAD:
 // broken or empty), st their (bond type << 8 | regime) for DM_NLIN, ci node
-// i's stiffness for DM_FIELD.  Two compile-time switches, so that the
-// common runs spend no instruction on what they do not use: CORR, the
-// per-slot stiffness corrections multiply the force (bond_force2/4); TENS,
-// a PMB bond breaks on !(s < s0) (cl/peridynamics.cl:78) instead of
-// !(|s| < s0) (peridynamics.pyx:133).
-template <int DM, int W, bool CORR, bool TENS>
+// i's stiffness for DM_FIELD.  Compile-time switches, so that the common
+// runs spend no instruction on what they do not use: CORR, the per-slot
+// stiffness corrections multiply the force (bond_force2/4); TENS, a PMB bond
+// breaks on !(s < s0) (cl/peridynamics.cl:78) instead of !(|s| < s0)
+// (peridynamics.pyx:133); CLA, the OpenCL stretch (records hold u; the bond
+// vector is (u_j - u_i) + xi) instead of the Cython one (records hold
+// r0 + u; the bond vector is r_j - r_i).
+template <int DM, int W, bool CORR, bool TENS, bool CLA>
 __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
                                       const double4 &r0i, const double4 &ri,
                                       double ci,
@@ -336,15 +361,22 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
             p3.x = *reinterpret_cast<const double *>(pe + 3 * PB);
             p3.y = 0.0;
         }
-        // spatial.pyx:20-31 with r = r0 + u: dr = r_j - r_i
-        dx[q] = p2.x - ri.x;
-        dy[q] = p2.y - ri.y;
-        dz[q] = p3.x - ri.z;
-        const double qq = (dx[q] * dx[q] + dy[q] * dy[q]) + dz[q] * dz[q];
         const double ex = p0.x - r0i.x;
         const double ey = p0.y - r0i.y;
         const double ez = p1.x - r0i.z;
-        const double q0 = (ex * ex + ey * ey) + ez * ez;
+        if (CLA) {
+            // cl/peridynamics.cl:65-71: xi_eta = (u_j - u_i) + xi
+            dx[q] = (p2.x - ri.x) + ex;
+            dy[q] = (p2.y - ri.y) + ey;
+            dz[q] = (p3.x - ri.z) + ez;
+        } else {
+            // spatial.pyx:20-31 with r = r0 + u: dr = r_j - r_i
+            dx[q] = p2.x - ri.x;
+            dy[q] = p2.y - ri.y;
+            dz[q] = p3.x - ri.z;
+        }
+        const double qq = sq3<CLA>(dx[q], dy[q], dz[q]);
+        const double q0 = sq3<CLA>(ex, ey, ez);
         double l, l0, h0;
         sqrt_core(qq, l, hl[q]);
         sqrt_core(q0, l0, h0);
@@ -370,15 +402,16 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
     for (int q = 0; q < W; ++q) {
         if (slow[q]) {
             // rare: near a threshold, or a square root outside the core's
-            // range: IEEE square roots and division (spatial.pyx:87-97)
+            // range: IEEE square roots and division (spatial.pyx:87-97,
+            // cl/peridynamics.cl:72-74)
             const char *pe = ldsb + 16u * ev[q];
             const double2 p0 = *reinterpret_cast<const double2 *>(pe);
             const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
-            const double qq = (dx[q] * dx[q] + dy[q] * dy[q]) + dz[q] * dz[q];
+            const double qq = sq3<CLA>(dx[q], dy[q], dz[q]);
             const double ex = p0.x - r0i.x;
             const double ey = p0.y - r0i.y;
             const double ez = p1.x - r0i.z;
-            const double q0 = (ex * ex + ey * ey) + ez * ez;
+            const double q0 = sq3<CLA>(ex, ey, ez);
             const double l = sqrt(qq), l0 = sqrt(q0);
             if (!(core_range(qq) & core_range(q0)))
                 hl[q] = 0.5 / l;
@@ -460,7 +493,7 @@ struct Chunk {
 // field and n-linear models would spill.)  Only the exact-division path -- a stretch
 // near a threshold, or a square root outside the core's range -- and
 // breaking branch.
-template <int DM, bool CORR, bool TENS>
+template <int DM, bool CORR, bool TENS, bool CLA>
 __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
                                        const double4 &r0i, const double4 &ri,
                                        double ci,
@@ -471,17 +504,19 @@ __device__ __forceinline__ void chunk4(const StepArgs &a, const char *ldsb,
 {
     const unsigned ev[4] = {ch.loc.x, ch.loc.y, ch.loc.z, ch.loc.w};
     const unsigned short st[4] = {ch.st.x, ch.st.y, ch.st.z, ch.st.w};
-    bonds<DM, 2, CORR, TENS>(a, ldsb, r0i, ri, ci, ev, st, s0, ts, tk, tp,
-                             fx, fy, fz, nbroken);
-    bonds<DM, 2, CORR, TENS>(a, ldsb, r0i, ri, ci, ev + 2, st + 2, s0 + 2,
-                             ts, tk, tp, fx, fy, fz, nbroken);
+    bonds<DM, 2, CORR, TENS, CLA>(a, ldsb, r0i, ri, ci, ev, st, s0, ts, tk,
+                                  tp, fx, fy, fz, nbroken);
+    bonds<DM, 2, CORR, TENS, CLA>(a, ldsb, r0i, ri, ci, ev + 2, st + 2,
+                                  s0 + 2, ts, tk, tp, fx, fy, fz, nbroken);
 }
 
 // DM: DM_PMB, DM_NLIN or DM_FIELD.  One workgroup per tile, LPN lanes per
 // node taking its 4-bond chunks round-robin and reducing with cross-lane
 // shuffles.  LDS (static, LDS_BYTES): the halo, four pieces of
 // piece_bytes<DM>() each, then for DM_NLIN the three damage-model tables.
-template <int DM, int LPN, bool CORR, bool TENS>
+// CLA: the slot layout's OpenCL arithmetic (records hold u, updates
+// contracted); otherwise the Cython path's (records hold r0 + u).
+template <int DM, int LPN, bool CORR, bool TENS, bool CLA>
 __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
                                                double fmag)
 {
@@ -593,8 +628,8 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         for (int c = sub; c < nch; c += LPN) {
             const Chunk nx = chunk_at(c + DEPTH * LPN);
             const size_t s0 = (size_t)(c0 + (long long)c * ti.nodes) * 4;
-            chunk4<DM, CORR, TENS>(a, ldsb, r0i, ri, ci, q0, s0, ts, tk, tp,
-                                   fx, fy, fz, nbroken);
+            chunk4<DM, CORR, TENS, CLA>(a, ldsb, r0i, ri, ci, q0, s0, ts, tk,
+                                        tp, fx, fy, fz, nbroken);
             q0 = q1;
             q1 = q2;
             q2 = q3;
@@ -615,8 +650,6 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
     if (sub == 0 && a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
         compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
                     a.n_neigh, (uint32_t)a.tnode[i], cnt);
-    if (a.exact)
-        return;  // breaks and lists only: k_exact_euler follows
 
     // the node's lanes share its three components (after the xor reduction
     // every lane holds the sums), so the per-node loads and stores coalesce
@@ -630,67 +663,74 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         // body_force differs from force only on a force-BC DOF, so only
         // those store it; reads take force elsewhere
         const bool fbc = (fl >> (3 + d)) & 1;
-        const double F = fbc ? bf + fmag * a.fbcv[k] : bf;
+        double F = bf;
+        if (fbc)
+            F = CLA ? __builtin_fma(fmag, a.fbcv[k], bf)
+                    : bf + fmag * a.fbcv[k];
         if (fbc)
             a.body_force[k] = bf;
         a.force[k] = F;
         const bool fixed = (fl >> d) & 1;
         double u = PRE ? pu : a.u[k];
         if (a.integrator == PD_EULER) {
-            // peridynamics.pyx:179-185, cl/euler.cl:21-23
-            u = fixed ? dmag * a.bcv[k] : u + a.dt * F;
+            // peridynamics.pyx:179-185; cl/euler.cl:21-23 (contracted)
+            const double un = CLA ? __builtin_fma(a.dt, F, u) : u + a.dt * F;
+            u = fixed ? dmag * a.bcv[k] : un;
         } else if (a.integrator == PD_EULER_CROMER) {
-            // cl/euler_cromer.cl:29-34
+            // cl/euler_cromer.cl:29-34 (contracted)
             double ud = PRE ? pud : a.ud[k];
             const double dens =
                 PRE ? pdens : (a.dens1 ? a.dens1[i] : a.dens[k]);
-            const double udd = (F - a.damping * ud) / dens;
-            ud = ud + udd * a.dt;
+            const double udd = __builtin_fma(-a.damping, ud, F) / dens;
+            ud = __builtin_fma(udd, a.dt, ud);
             a.udd[k] = udd;
             a.ud[k] = ud;
-            u = fixed ? dmag * a.bcv[k] : u + a.dt * ud;
+            u = fixed ? dmag * a.bcv[k] : __builtin_fma(a.dt, ud, u);
         } else {
-            // cl/velocity_verlet.cl:29-35
+            // cl/velocity_verlet.cl:29-35 (contracted)
             const double hdt = a.dt / 2;
-            const double ud1 =
-                (PRE ? pud : a.ud[k]) + hdt * (PRE ? pudd : a.udd[k]);
+            const double ud1 = __builtin_fma(hdt, PRE ? pudd : a.udd[k],
+                                             PRE ? pud : a.ud[k]);
             const double dens =
                 PRE ? pdens : (a.dens1 ? a.dens1[i] : a.dens[k]);
-            const double udd1 = (F - a.damping * ud1) / dens;
-            const double ud = ud1 + hdt * udd1;
+            const double udd1 = __builtin_fma(-a.damping, ud1, F) / dens;
+            const double ud = __builtin_fma(hdt, udd1, ud1);
             a.ud[k] = ud;
             a.udd[k] = udd1;
-            u = fixed ? dmag * a.bcv[k] : u + a.dt * (ud + hdt * udd1);
+            u = fixed ? dmag * a.bcv[k]
+                      : __builtin_fma(a.dt, __builtin_fma(hdt, udd1, ud), u);
         }
         a.u_out[k] = u;
-        // r = r0 + u for the next step; the record's c (w) is static
+        // the bond loop's position for the next step: r0 + u (compact) or
+        // u (slot); the record's c (w) is static
         const double r0d = d == 0 ? r0i.x : (d == 1 ? r0i.y : r0i.z);
-        reinterpret_cast<double *>(&a.rec_new[i].b)[d] = r0d + u;
+        reinterpret_cast<double *>(&a.rec_new[i].b)[d] = CLA ? u : r0d + u;
     }
 }
 
 // The stretch pipeline of bond<DM_PMB>, one bond per thread, for checking:
-// l, l0 (bitwise those of spatial.pyx), the working s (2*sh), the exact
-// s = (l - l0)/l0 and the break decision.
-__global__ void k_bond_check(int n, const double *ri, const double *rj,
-                             const double *r0i, const double *r0j,
-                             double s0, unsigned absmask, double *l_out,
-                             double *l0_out, double *s_out, double *s_exact,
-                             int *broken)
+// l, l0 (bitwise those of spatial.pyx, or of cl/peridynamics.cl:65-75 as
+// contracted with cla), the working s (2*sh), the exact s = (l - l0)/l0 and
+// the break decision.  With cla, ri and rj are the displacements u_i, u_j
+// (the OpenCL kernels form the bond from u); otherwise positions r0 + u.
+template <bool CLA>
+__device__ void bond_check_one(const double *ri, const double *rj,
+                               const double *r0i, const double *r0j,
+                               double s0, unsigned absmask, double &l,
+                               double &l0, double &s, double &se, int &brk)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    const size_t b = 3 * (size_t)t;
-    const double dx = rj[b] - ri[b];
-    const double dy = rj[b + 1] - ri[b + 1];
-    const double dz = rj[b + 2] - ri[b + 2];
-    const double q = (dx * dx + dy * dy) + dz * dz;
-    const double ex = r0j[b] - r0i[b];
-    const double ey = r0j[b + 1] - r0i[b + 1];
-    const double ez = r0j[b + 2] - r0i[b + 2];
-    const double q0 = (ex * ex + ey * ey) + ez * ez;
-    double l, hl, l0, h0;
+    const double ex = r0j[0] - r0i[0];
+    const double ey = r0j[1] - r0i[1];
+    const double ez = r0j[2] - r0i[2];
+    double dx = rj[0] - ri[0], dy = rj[1] - ri[1], dz = rj[2] - ri[2];
+    if (CLA) {
+        dx = dx + ex;
+        dy = dy + ey;
+        dz = dz + ez;
+    }
+    const double q = sq3<CLA>(dx, dy, dz);
+    const double q0 = sq3<CLA>(ex, ey, ez);
+    double hl, h0;
     sqrt_core(q, l, hl);
     sqrt_core(q0, l0, h0);
     double dl = l - l0;
@@ -707,18 +747,42 @@ __global__ void k_bond_check(int n, const double *ri, const double *rj,
         sh = (dl / l0) * 0.5;
         as = sign_mask(sh, absmask);
     }
+    s = sh + sh;
+    se = dl / l0;
+    brk = !(as < s0h);
+}
+
+__global__ void k_bond_check(int n, int cla, const double *ri,
+                             const double *rj, const double *r0i,
+                             const double *r0j, double s0, unsigned absmask,
+                             double *l_out, double *l0_out, double *s_out,
+                             double *s_exact, int *broken)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const size_t b = 3 * (size_t)t;
+    double l, l0, s, se;
+    int brk;
+    if (cla)
+        bond_check_one<true>(ri + b, rj + b, r0i + b, r0j + b, s0, absmask,
+                             l, l0, s, se, brk);
+    else
+        bond_check_one<false>(ri + b, rj + b, r0i + b, r0j + b, s0, absmask,
+                              l, l0, s, se, brk);
     l_out[t] = l;
     l0_out[t] = l0;
-    s_out[t] = sh + sh;
-    s_exact[t] = dl / l0;
-    broken[t] = !(as < s0h);
+    s_out[t] = s;
+    s_exact[t] = se;
+    broken[t] = brk;
 }
 
 // Records of both ping-pong buffers from r0, V, u and c: a = (r0, wscale V),
-// b = (r0 + u, v) with v = 2 c V for a field c, else 0.  The weights are
-// the factors the bond loop multiplies by: wscale = 4k for the scalar
-// model, 2 for the field model, 1 for the n-linear model.
-__global__ void k_records(int n, double wscale, const double *r0,
+// b = (r0 + u, v) -- (u, v) with ucla, the slot layout's OpenCL stretch --
+// with v = 2 c V for a field c, else 0.  The weights are the factors the
+// bond loop multiplies by: wscale = 4k for the scalar model, 2 for the field
+// model, 1 for the n-linear model.
+__global__ void k_records(int n, double wscale, int ucla, const double *r0,
                           const double *vol, const double *u, const double *c,
                           NodeRec *rec0, NodeRec *rec1)
 {
@@ -728,8 +792,10 @@ __global__ void k_records(int n, double wscale, const double *r0,
     const size_t b = 3 * (size_t)i;
     NodeRec r;
     r.a = make_double4(r0[b], r0[b + 1], r0[b + 2], vol[i] * wscale);
-    r.b = make_double4(r0[b] + u[b], r0[b + 1] + u[b + 1], r0[b + 2] + u[b + 2],
-                       c ? 2.0 * c[i] * vol[i] : 0.0);
+    const double v = c ? 2.0 * c[i] * vol[i] : 0.0;
+    r.b = ucla ? make_double4(u[b], u[b + 1], u[b + 2], v)
+               : make_double4(r0[b] + u[b], r0[b + 1] + u[b + 1],
+                              r0[b + 2] + u[b + 2], v);
     rec0[i] = r;
     rec1[i] = r;
 }
@@ -894,139 +960,289 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
         atomicAdd(total, part[0]);
 }
 
-// --- exact-order Euler step (verification mode of the Cython path) --------
+// --- exact-order Euler step of the Cython path ----------------------------
 //
 // peridynamics.pyx:34-99 (bond_force) visits i ascending and, for each live
 // j > i of i's row in row order, adds f V_j to F_i and subtracts f V_i from
 // F_j.  So F_m is the left fold, from +0.0, of first the bonds to live
-// k < m in ascending k (subtracting (fn (r_m - r_k)) V_k), then the bonds
-// to the live j > m of m's current row in row order (adding
-// (fn (r_j - r_m)) V_j), then the force boundary condition.  Each thread
-// folds one node in that order with the reference's operations (IEEE square
-// roots and divisions, no contraction): after k_step has broken the bonds
-// and compacted the rows (peridynamics.pyx:102-156), the force and the
-// Euler update (:159-185) equal the Cython path's bitwise.  Slots hold each
-// node's bonds ascending by j (caller ids), so the lower neighbours come
-// first; inv maps caller ids to internal ones.  Assumes symmetric lists, as
-// the model builds them.
-__device__ __forceinline__ double cython_fn(const double *ri,
-                                            const double *rj,
-                                            const double *r0i,
-                                            const double *r0j, double k)
+// k < m in ascending k, then the bonds to the live j > m of m's current row
+// in row order, then the force boundary condition.  Every term is
+// (fn (r_j - r_m)) V_j with fn = ((l - l0)/l0 k)/l, l = |r_j - r_m|,
+// l0 = |r0_j - r0_m| (spatial.pyx:20-31, 87-97): for a lower neighbour the
+// reference subtracts (fn (r_m - r_k)) V_k, which is the same double,
+// negated, as r_m - r_k is the exact negation of r_k - r_m.  The fold order
+// depends only on node m's own row, so each tile folds its own nodes.
+//
+// A node's fold list is its lower slots in ascending order (slot order:
+// rows are sorted by caller id; a dead slot adds nothing), then its upper
+// neighbours in current row order (uord: slot index << 16 | halo loc,
+// rebuilt by compact_row after every break of the node).  The node's four
+// lanes take four consecutive positions per round: each evaluates its bond
+// with the reference's operations -- bitwise square roots, IEEE divisions,
+// no contraction -- decides the break from that exact stretch
+// (peridynamics.pyx:133), and writes its three force terms to the node's
+// 96-byte LDS window; lanes 0-2 then add the four terms of their component
+// in order.  Breaking is applied before the force, as in Euler.__call__: a
+// bond that breaks adds nothing, and when any bond of the node breaks the
+// row is compacted (peridynamics.pyx:102-156) and the upper part, whose
+// order the compaction may change, is folded again from the sum of the
+// lower part.  Assumes symmetric neighbour lists without repeats, as the
+// model builds them.
+
+// LDS of the exact step: the halo (pieces of EXACT_CAP entries), then the
+// fold windows (12 doubles per node, tiles of up to EXACT_NODES nodes).
+constexpr int EXACT_NODES = 128;
+constexpr int EXACT_CAP = (int)((LDS_BYTES - EXACT_NODES * 96u) / 64u);
+
+// Slot index of caller id j among node i's slots (keys ascending).
+__device__ int slot_index(const int *gidx, long long c0, int stride, int cnt,
+                          int j)
 {
-    // spatial.pyx:20-31 (ceuclid) and :87-97 (cstrain2)
-    double d0 = rj[0] - ri[0], d1 = rj[1] - ri[1], d2 = rj[2] - ri[2];
-    d0 = d0 * d0;
-    d1 = d1 * d1;
-    d2 = d2 * d2;
-    const double l = sqrt((d0 + d1) + d2);
-    double e0 = r0j[0] - r0i[0], e1 = r0j[1] - r0i[1], e2 = r0j[2] - r0i[2];
-    e0 = e0 * e0;
-    e1 = e1 * e1;
-    e2 = e2 * e2;
-    const double l0 = sqrt((e0 + e1) + e2);
-    const double strain = (l - l0) / l0;
-    // peridynamics.pyx:75-80
-    double fn = strain * k;
-    fn = fn / l;
-    return fn;
+    int lo = 0, hi = cnt - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int key =
+            (int)(ld_slot(&gidx[slot_of(c0, stride, mid)]) & KEYM);
+        if (key == j)
+            return mid;
+        if (key < j)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
 }
 
-__global__ void k_exact_euler(StepArgs a, const int *inv, const double *vol,
-                              double k, double dmag, double fmag)
+// The upper list of node p (caller id i) in the order of its current
+// reference-layout row.
+__device__ __noinline__ void rebuild_uord(const int *gidx,
+                                          const unsigned short *loc,
+                                          unsigned *uord, int *nup,
+                                          long long c0, int stride,
+                                          const int *row, int n, uint32_t i,
+                                          int cnt, int p)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n)
+    int c = 0;
+    for (int q = 0; q < n; ++q) {
+        const int j = row[q];
+        if (j > (int)i) {
+            const int k = slot_index(gidx, c0, stride, cnt, j);
+            uord[slot_of(c0, stride, c++)] =
+                ((unsigned)k << 16) | loc[slot_of(c0, stride, k)];
+        }
+    }
+    nup[p] = c;
+}
+
+// The exact bond evaluation: the term's components and the stretch.
+struct ExactTerm {
+    double t0, t1, t2, s;
+};
+
+__device__ __forceinline__ ExactTerm exact_term(const char *ldsb, unsigned e,
+                                                const double4 &r0m,
+                                                const double4 &rm, double k)
+{
+    constexpr unsigned PB = 16u * EXACT_CAP;
+    const char *pe = ldsb + 16u * e;
+    const double2 p0 = *reinterpret_cast<const double2 *>(pe);
+    const double2 p1 = *reinterpret_cast<const double2 *>(pe + PB);
+    const double2 p2 = *reinterpret_cast<const double2 *>(pe + 2 * PB);
+    const double p3 = *reinterpret_cast<const double *>(pe + 3 * PB);
+    // spatial.pyx:20-31: squares of r_j - r_m summed left to right
+    const double dx = p2.x - rm.x, dy = p2.y - rm.y, dz = p3 - rm.z;
+    const double q = (dx * dx + dy * dy) + dz * dz;
+    const double ex = p0.x - r0m.x, ey = p0.y - r0m.y, ez = p1.x - r0m.z;
+    const double q0 = (ex * ex + ey * ey) + ez * ez;
+    double l, l0, h;
+    sqrt_core(q, l, h);
+    sqrt_core(q0, l0, h);
+    if (!(core_range(q) & core_range(q0))) {
+        l = sqrt(q);
+        l0 = sqrt(q0);
+    }
+    ExactTerm t;
+    // spatial.pyx:87-97, peridynamics.pyx:75-89
+    t.s = (l - l0) / l0;
+    double fn = t.s * k;
+    fn = fn / l;
+    const double v = p1.y;  // V_j
+    t.t0 = (fn * dx) * v;
+    t.t1 = (fn * dy) * v;
+    t.t2 = (fn * dz) * v;
+    return t;
+}
+
+// The fold window of a node: [component][position of the round].
+__device__ __forceinline__ void window_put(double *win, int sub,
+                                          const ExactTerm &t)
+{
+    win[sub] = t.t0;
+    win[4 + sub] = t.t1;
+    win[8 + sub] = t.t2;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void window_done()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One workgroup per tile of at most EXACT_NODES nodes, four lanes per node
+// (block 4 * tile_nodes), on k_step's tiles and halo staging.  Records hold
+// (r0, V | r0 + u, 0).
+__global__ __launch_bounds__(512) void k_exact_step(StepArgs a, double dmag,
+                                                    double fmag)
+{
+    __shared__ double2 lds[LDS_BYTES / sizeof(double2)];
+    constexpr int LPN = 4;
+    constexpr unsigned PB = 16u * EXACT_CAP;
+    const int b = blockIdx.x;
+    const int tile = (b % XCDS) * a.tiles_per_xcd + b / XCDS;
+    if (tile >= a.ntiles)
         return;
-    const uint32_t m = (uint32_t)a.tnode[p];
-    const NodeRec own = a.rec_old[p];
-    const double rm[3] = {own.b.x, own.b.y, own.b.z};
-    const double r0m[3] = {own.a.x, own.a.y, own.a.z};
-    double F[3] = {0.0, 0.0, 0.0};
-    // neighbours EB at a time: their index and record loads are issued
-    // together, then they are folded in order (g < 0: not folded)
-    constexpr int EB = 4;
-    const int c = a.cnt[p];
-    bool lower = true;
-    for (int q0 = 0; q0 < c && lower; q0 += EB) {
-        int g[EB];
-#pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            const uint32_t e =
-                q0 + t < c ? (uint32_t)a.gidx[slot_of(a.chunk0[p],
-                                                      a.stride[p], q0 + t)]
-                           : KEYM;
-            const uint32_t key = e & KEYM;
-            g[t] = key < m ? ((e & DEAD) ? -1 : inv[key]) : -2;
-        }
-        NodeRec o[EB];
-        double v[EB];
-#pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            if (g[t] >= 0) {
-                o[t] = a.rec_old[g[t]];
-                v[t] = vol[g[t]];
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            if (g[t] == -2)
-                lower = false;  // keys ascend: the rest are above m
-            if (!lower || g[t] < 0)
-                continue;
-            const double rk[3] = {o[t].b.x, o[t].b.y, o[t].b.z};
-            const double r0k[3] = {o[t].a.x, o[t].a.y, o[t].a.z};
-            const double fn = cython_fn(rk, rm, r0k, r0m, k);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-                F[d] = F[d] - (fn * (rm[d] - rk[d])) * v[t];
+    const TileInfo ti = a.tiles[tile];
+    char *ldsb = reinterpret_cast<char *>(lds);
+    const int l = threadIdx.x / LPN;
+    const int sub = threadIdx.x % LPN;
+    const bool valid = l < ti.nodes;
+    const uint32_t i = (uint32_t)(ti.node_off + (valid ? l : 0));
+    const long long c0 = ti.chunk_off + (valid ? l : 0);
+    double4 r0m = make_double4(0.0, 0.0, 0.0, 0.0), rm = r0m;
+    int cnt = 0, nlow = 0, nup = 0, fl = 0;
+    double pu = 0.0;
+    if (valid) {
+        r0m = a.rec_old[i].a;
+        rm = a.rec_old[i].b;
+        cnt = a.cnt[i];
+        nlow = a.nlow[i];
+        nup = a.nup[i];
+        if (sub < 3) {
+            fl = a.flags[i];
+            pu = a.u[3 * (size_t)i + sub];
         }
     }
-    const int *row = a.nlist_ref + (size_t)m * a.K;
-    const int nn = a.n_neigh[m];
-    for (int t0 = 0; t0 < nn; t0 += EB) {
-        int g[EB];
+    // stage the halo (as k_step)
+    constexpr int SP = 10;
+    const int np = 4 * ti.halo;
+    for (int x0 = threadIdx.x; x0 < np; x0 += SP * blockDim.x) {
+        double2 v[SP];
 #pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            const int j = t0 + t < nn ? row[t0 + t] : -1;
-            g[t] = j > (int)m ? inv[j] : -1;
-        }
-        NodeRec o[EB];
-        double v[EB];
-#pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            if (g[t] >= 0) {
-                o[t] = a.rec_old[g[t]];
-                v[t] = vol[g[t]];
-            }
+        for (int k = 0; k < SP; ++k) {
+            const int x = min(x0 + k * (int)blockDim.x, np - 1);
+            const int g = a.halo[ti.halo_off + (x >> 2)];
+            v[k] = reinterpret_cast<const double2 *>(a.rec_old + g)[x & 3];
         }
 #pragma unroll
-        for (int t = 0; t < EB; ++t) {
-            if (g[t] < 0)
-                continue;
-            const double rj[3] = {o[t].b.x, o[t].b.y, o[t].b.z};
-            const double r0j[3] = {o[t].a.x, o[t].a.y, o[t].a.z};
-            const double fn = cython_fn(rm, rj, r0m, r0j, k);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-                F[d] = F[d] + (fn * (rj[d] - rm[d])) * v[t];
+        for (int k = 0; k < SP; ++k) {
+            const int x = x0 + k * (int)blockDim.x;
+            if (x < np)
+                *reinterpret_cast<double2 *>(
+                    ldsb + (x & 3) * PB + 16u * (unsigned)((x >> 2) + 1)) =
+                    v[k];
         }
     }
-    const int fl = a.flags[p];
-    const size_t b3 = 3 * (size_t)p;
+    if (threadIdx.x < 4)
+        *reinterpret_cast<double2 *>(ldsb + threadIdx.x * PB) =
+            make_double2(0.0, 0.0);
+    __syncthreads();
+
+    double *win = reinterpret_cast<double *>(ldsb + 4 * PB) + 12 * l;
+    const double s0 = 2.0 * a.s0h;
+    double F = 0.0, Flow = 0.0;  // lane d < 3: component d
+    int nbroken = 0;
+    if (valid) {
+        const int total = nlow + nup;
+        for (int r = 0; r < total; r += LPN) {
+            const int pos = r + sub;
+            unsigned e = LOC_DEAD;
+            size_t slot = 0;
+            if (pos < nlow) {
+                slot = slot_of(c0, ti.nodes, pos);
+                e = a.loc[slot];
+            } else if (pos < total) {
+                const unsigned u = a.uord[slot_of(c0, ti.nodes, pos - nlow)];
+                slot = slot_of(c0, ti.nodes, (int)(u >> 16));
+                e = u & 0xffffu;
+            }
+            ExactTerm t = {0.0, 0.0, 0.0, 0.0};
+            if (e != LOC_DEAD) {
+                t = exact_term(ldsb, e, r0m, rm, a.kx);
+                const double as = sign_mask(t.s, a.absmask);
+                if (!(as < s0)) {  // peridynamics.pyx:133
+                    a.loc[slot] = LOC_DEAD;
+                    a.gidx[slot] |= (int)(DEAD | NEWB);
+                    ++nbroken;
+                    t.t0 = t.t1 = t.t2 = 0.0;
+                }
+            }
+            window_put(win, sub, t);
+            if (sub < 3) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const size_t kd = b3 + d;
+                for (int q = 0; q < LPN; ++q) {
+                    if (r + q == nlow)
+                        Flow = F;
+                    F = F + win[4 * sub + q];
+                }
+            }
+            window_done();
+        }
+        if (nlow >= total)
+            Flow = F;
+    }
+#pragma unroll
+    for (int off = LPN / 2; off > 0; off >>= 1)
+        nbroken += __shfl_xor(nbroken, off, LPN);
+    if (!valid)
+        return;
+    if (nbroken > 0) {
+        // compaction, then the upper part again in the new row order
+        const uint32_t m = (uint32_t)a.tnode[i];
+        if (sub == 0) {
+            compact_row(a.gidx, c0, ti.nodes, a.nlist_ref, a.K, a.n_neigh, m,
+                        cnt);
+            rebuild_uord(a.gidx, a.loc, a.uord, a.nup, c0, ti.nodes,
+                         a.nlist_ref + (size_t)m * a.K, a.n_neigh[m], m, cnt,
+                         (int)i);
+        }
+        __threadfence();
+        const int nup2 = ld_slot(&a.nup[i]);
+        F = Flow;
+        for (int r = 0; r < nup2; r += LPN) {
+            const int pos = r + sub;
+            ExactTerm t = {0.0, 0.0, 0.0, 0.0};
+            if (pos < nup2) {
+                const unsigned u = ld_slot(reinterpret_cast<const int *>(
+                    &a.uord[slot_of(c0, ti.nodes, pos)]));
+                t = exact_term(ldsb, u & 0xffffu, r0m, rm, a.kx);
+            }
+            window_put(win, sub, t);
+            if (sub < 3) {
+#pragma unroll
+                for (int q = 0; q < LPN; ++q)
+                    F = F + win[4 * sub + q];
+            }
+            window_done();
+        }
+    }
+    if (sub < 3) {
         // peridynamics.pyx:93-97, then :179-185
-        const bool fbc = (fl >> (3 + d)) & 1;
-        const double Fd = fbc ? F[d] + fmag * a.fbcv[kd] : F[d];
-        if (fbc)
-            a.body_force[kd] = F[d];
-        a.force[kd] = Fd;
-        const double u =
-            (fl >> d) & 1 ? dmag * a.bcv[kd] : a.u[kd] + a.dt * Fd;
-        a.u_out[kd] = u;
-        reinterpret_cast<double *>(&a.rec_new[p].b)[d] = r0m[d] + u;
+        const size_t k = 3 * (size_t)i + sub;
+        const bool fbc = (fl >> (3 + sub)) & 1;
+        double Fd = F;
+        if (fbc) {
+            Fd = F + fmag * a.fbcv[k];
+            a.body_force[k] = F;
+        }
+        a.force[k] = Fd;
+        const double u = (fl >> sub) & 1 ? dmag * a.bcv[k] : pu + a.dt * Fd;
+        a.u_out[k] = u;
+        const double r0d = sub == 0 ? r0m.x : (sub == 1 ? r0m.y : r0m.z);
+        reinterpret_cast<double *>(&a.rec_new[i].b)[sub] = r0d + u;
     }
 }
 
@@ -1073,6 +1289,19 @@ __global__ void k_unperm(int n, int w, const int *tnode, const double *src,
     const size_t c = (size_t)tnode[p] * w, q = (size_t)p * w;
     for (int d = 0; d < w; ++d)
         dst[c + d] = src[q + d];
+}
+
+// A per-node array in the library's node order from the caller's:
+// dst[p] = src[tnode[p]], w values per node.
+__global__ void k_perm(int n, int w, const int *tnode, const double *src,
+                       double *dst)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const size_t c = (size_t)tnode[p] * w, q = (size_t)p * w;
+    for (int d = 0; d < w; ++d)
+        dst[q + d] = src[c + d];
 }
 
 // brk at the start of a recording: -1 for bonds already broken, else "never".
@@ -1341,11 +1570,12 @@ struct pd_sim {
     int nregimes = 1, nbond_types = 1;
     double k = 0.0, s0 = 0.0;
     int cur = 0;
-    bool exact = false;  // exact-order Euler step (k_exact_euler)
+    bool exact = false;  // exact-order Euler step (k_exact_step)
     // a replay's u_n and u_{n+1} (slots of the segment buffer); null: u
     const double *u_in = nullptr;
     double *u_out = nullptr;
-    DevBuf<int> inv;     // caller id -> internal id, for the exact step
+    DevBuf<int> nlow, nup;  // exact step: lower slots, upper-list length
+    DevBuf<unsigned> uord;  // exact step: upper lists in row order
     DevBuf<double> wbuf;  // a per-node array reordered for download
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1364,6 +1594,7 @@ struct pd_sim {
     DevBuf<double> dens1;
     DevBuf<double> r0, vol, u, ud, udd, force, body_force, bcv, fbcv, dens,
         damage, cnode, corr, k_tab, s_tab, pc_tab;
+    DevBuf<double> cnode_caller;  // a field given on the device (caller order)
     DevBuf<int> flags, family, cnt, tnode, halo, gidx, stride, nlist_ref,
         n_neigh, out_nlist, out_nneigh;
     DevBuf<long long> chunk0;
@@ -1519,30 +1750,56 @@ StepArgs make_args(pd_sim *s)
     a.cnode = s->cnode.p;
     a.brk = s->adj_recording ? s->brk.p : nullptr;
     a.step = s->step_counter;
-    a.exact = s->exact ? 1 : 0;
+    a.nlow = s->nlow.p;
+    a.nup = s->nup.p;
+    a.uord = s->uord.p;
+    a.kx = s->k;
     return a;
 }
 
-template <int DM, int LPN>
-hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
+template <int DM, int LPN, bool CLA>
+hipError_t launch_cla(pd_sim *s, const StepArgs &a, double dmag, double fmag)
 {
     const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
     const unsigned block = (unsigned)(s->tile_nodes * LPN);
     // the n-linear model breaks by regimes, not by the sign mode
     const bool tens = DM != DM_NLIN && a.absmask == 0xffffffffu;
     if (a.has_corr && tens)
-        hipLaunchKernelGGL((k_step<DM, LPN, true, DM != DM_NLIN>), dim3(grid),
-                           dim3(block), 0, s->stream, a, dmag, fmag);
+        hipLaunchKernelGGL((k_step<DM, LPN, true, DM != DM_NLIN, CLA>),
+                           dim3(grid), dim3(block), 0, s->stream, a, dmag,
+                           fmag);
     else if (a.has_corr)
-        hipLaunchKernelGGL((k_step<DM, LPN, true, false>), dim3(grid),
+        hipLaunchKernelGGL((k_step<DM, LPN, true, false, CLA>), dim3(grid),
                            dim3(block), 0, s->stream, a, dmag, fmag);
     else if (tens)
-        hipLaunchKernelGGL((k_step<DM, LPN, false, DM != DM_NLIN>),
+        hipLaunchKernelGGL((k_step<DM, LPN, false, DM != DM_NLIN, CLA>),
                            dim3(grid), dim3(block), 0, s->stream, a, dmag,
                            fmag);
     else
-        hipLaunchKernelGGL((k_step<DM, LPN, false, false>), dim3(grid),
+        hipLaunchKernelGGL((k_step<DM, LPN, false, false, CLA>), dim3(grid),
                            dim3(block), 0, s->stream, a, dmag, fmag);
+    return hipGetLastError();
+}
+
+// The slot layout steps with the OpenCL arithmetic; the compact layout (the
+// Cython path: scalar stiffness, no corrections) with the Cython one.
+template <int DM, int LPN>
+hipError_t launch_one(pd_sim *s, const StepArgs &a, double dmag, double fmag)
+{
+    if (a.layout == PD_LAYOUT_SLOT)
+        return launch_cla<DM, LPN, true>(s, a, dmag, fmag);
+    const unsigned grid = (unsigned)XCDS * a.tiles_per_xcd;
+    const unsigned block = (unsigned)(s->tile_nodes * LPN);
+    if (DM != DM_PMB || a.has_corr)
+        return hipErrorInvalidValue;  // rejected before launch
+    if (a.absmask == 0xffffffffu)
+        hipLaunchKernelGGL((k_step<DM_PMB, LPN, false, true, false>),
+                           dim3(grid), dim3(block), 0, s->stream, a, dmag,
+                           fmag);
+    else
+        hipLaunchKernelGGL((k_step<DM_PMB, LPN, false, false, false>),
+                           dim3(grid), dim3(block), 0, s->stream, a, dmag,
+                           fmag);
     return hipGetLastError();
 }
 
@@ -1643,30 +1900,37 @@ int launch_step(pd_sim *s, double dmag, double fmag)
         s->cur = 1 - s->cur;
         return PD_OK;
     }
+    if (s->layout == PD_LAYOUT_COMPACT && dm_of(s) != DM_PMB)
+        return fail(PD_E_STATE, "the compact (Cython) layout steps the "
+                                "scalar stiffness model only");
     if (s->exact && (s->layout != PD_LAYOUT_COMPACT ||
                      s->integrator != PD_EULER || dm_of(s) != DM_PMB ||
-                     s->has_corr || !s->inv.p))
+                     s->has_corr || !s->uord.p))
         return fail(PD_E_STATE, "the exact-order step needs the compact "
                                 "layout, the Euler integrator and the scalar "
                                 "model without corrections");
+    if (s->exact && (s->lpn != 4 || s->tile_nodes > EXACT_NODES))
+        return fail(PD_E_STATE, "the exact-order step runs 4 lanes per node "
+                                "and tiles of at most 128 nodes");
     const StepArgs a = make_args(s);
     hipError_t e;
-    switch (dm_of(s)) {
-    case DM_NLIN:
-        e = launch_dm<DM_NLIN>(s, a, dmag, fmag);
-        break;
-    case DM_FIELD:
-        e = launch_dm<DM_FIELD>(s, a, dmag, fmag);
-        break;
-    default:
-        e = launch_dm<DM_PMB>(s, a, dmag, fmag);
-        break;
-    }
-    if (e == hipSuccess && s->exact) {
-        hipLaunchKernelGGL(k_exact_euler, dim3((s->n + 255) / 256),
-                           dim3(256), 0, s->stream, a, s->inv.p, s->vol.p,
-                           s->k, dmag, fmag);
+    if (s->exact) {
+        hipLaunchKernelGGL(k_exact_step, dim3(XCDS * a.tiles_per_xcd),
+                           dim3(4 * s->tile_nodes), 0, s->stream, a, dmag,
+                           fmag);
         e = hipGetLastError();
+    } else {
+        switch (dm_of(s)) {
+        case DM_NLIN:
+            e = launch_dm<DM_NLIN>(s, a, dmag, fmag);
+            break;
+        case DM_FIELD:
+            e = launch_dm<DM_FIELD>(s, a, dmag, fmag);
+            break;
+        default:
+            e = launch_dm<DM_PMB>(s, a, dmag, fmag);
+            break;
+        }
     }
     if (e != hipSuccess)
         return fail(PD_E_HIP, std::string("step kernel launch: ") +
@@ -1691,25 +1955,18 @@ int upload_tips(pd_sim *s)
     return PD_OK;
 }
 
-// inv[caller id] = internal id for the current tiling.
-int upload_inv(pd_sim *s)
-{
-    std::vector<int> inv(s->n);
-    for (int p = 0; p < s->n; ++p)
-        inv[s->perm[p]] = p;
-    HIPCHK(upload(s->inv, inv.data(), inv.size()));
-    return PD_OK;
-}
-
 // (Re)write both record buffers from r0, V, the device u and c.
 int refresh_records(pd_sim *s)
 {
     const int blocks = (s->n + 255) / 256;
     const int dm = dm_of(s);
+    // the exact-order step reads V itself
     const double wscale =
-        dm == DM_PMB ? 4.0 * s->k : (dm == DM_FIELD ? 2.0 : 1.0);
+        s->exact ? 1.0
+                 : (dm == DM_PMB ? 4.0 * s->k : (dm == DM_FIELD ? 2.0 : 1.0));
     hipLaunchKernelGGL(k_records, dim3(blocks), dim3(256), 0, s->stream, s->n,
-                       wscale, s->r0.p, s->vol.p, s->u.p,
+                       wscale, s->layout == PD_LAYOUT_SLOT ? 1 : 0, s->r0.p,
+                       s->vol.p, s->u.p,
                        s->has_cnode ? s->cnode.p : nullptr, s->rec[0].p,
                        s->rec[1].p);
     HIPCHK(hipGetLastError());
@@ -2039,7 +2296,11 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     // tiles: greedy over the Morton order, capped by tile_nodes and by the
     // LDS halo capacity
     // entry 0 of a staged halo is the zero record of dead slots
-    s->hmax = (nlin ? LDS_CAP_NLIN : LDS_CAP_PMB) - 1;
+    // the compact layout's tiles leave room for the exact step's windows
+    s->hmax = (nlin ? LDS_CAP_NLIN
+                    : (s->layout == PD_LAYOUT_COMPACT ? EXACT_CAP
+                                                      : LDS_CAP_PMB)) -
+              1;
     const std::vector<int> order = morton_order(
         N, s->h_coords.data(), s->h_group.empty() ? nullptr : s->h_group.data());
     std::vector<int> stamp(N, -1);
@@ -2128,6 +2389,9 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         state.assign(S, 0);
     std::vector<long long> chunk0(N);
     std::vector<int> stride(N);
+    const bool compact = s->layout == PD_LAYOUT_COMPACT;
+    std::vector<int> nlow_h(compact ? N : 0), nup_h(compact ? N : 0);
+    std::vector<unsigned> uord_h(compact ? S : 0, 0u);
     std::atomic<int> halo_mismatch{0};
     run_parallel(ntiles, [&](int lo, int hi) {
         std::vector<int> lmap(N, -1);  // internal id -> halo position
@@ -2168,6 +2432,26 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                     if (nlin)
                         state[sl] = (unsigned short)(
                             (s->h_btypes[src] << 8) | (regimes ? regimes[src] : 0));
+                }
+                if (compact) {
+                    // the exact step's lists: lower slots, and the upper
+                    // neighbours in the row's order
+                    const int p = ti.node_off + l;
+                    const int *rb = rj.data() + rstart[i];
+                    nlow_h[p] = (int)(std::lower_bound(rb, rb + cnt[i], i) - rb);
+                    const int *row = nlist + (size_t)i * K;
+                    int c = 0;
+                    for (int q = 0; q < cnt[i]; ++q) {
+                        const int j = row[q];
+                        if (j <= i)
+                            continue;
+                        const int k =
+                            (int)(std::lower_bound(rb, rb + cnt[i], j) - rb);
+                        uord_h[slot_of(ti.chunk_off + l, ti.nodes, c++)] =
+                            ((unsigned)k << 16) |
+                            loc[slot_of(ti.chunk_off + l, ti.nodes, k)];
+                    }
+                    nup_h[p] = c;
                 }
             }
             for (int h : hl)
@@ -2220,9 +2504,18 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
             chk(upload_perm(s->dens1, s->perm, d1.data(), 1), "densities");
         }
     }
-    if (s->has_cnode)
+    if (s->has_cnode && !s->h_cnode.empty()) {
         chk(upload_perm(s->cnode, s->perm, s->h_cnode.data(), 1),
             "node stiffness");
+    } else if (s->has_cnode) {
+        // given on the device: into the new node order there
+        chk(s->cnode.alloc(N), "node stiffness");
+        if (rc == PD_OK)
+            hipLaunchKernelGGL(k_perm, dim3((N + 255) / 256), dim3(256), 0,
+                               s->stream, (int)N, 1, s->tnode.p,
+                               s->cnode_caller.p, s->cnode.p);
+        chk(hipGetLastError(), "node stiffness");
+    }
     chk(upload(s->chunk0, chunk0.data(), (size_t)N), "chunk0");
     chk(upload(s->stride, stride.data(), (size_t)N), "stride");
     chk(upload(s->loc, loc.data(), loc.size()), "loc");
@@ -2239,9 +2532,15 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     if (s->layout == PD_LAYOUT_COMPACT) {
         chk(upload(s->nlist_ref, nlist, (size_t)N * K), "nlist");
         chk(upload(s->n_neigh, n_neigh, (size_t)N), "n_neigh");
+        chk(upload(s->nlow, nlow_h.data(), nlow_h.size()), "exact lists");
+        chk(upload(s->nup, nup_h.data(), nup_h.size()), "exact lists");
+        chk(upload(s->uord, uord_h.data(), uord_h.size()), "exact lists");
     } else {
         s->nlist_ref.release();
         s->n_neigh.release();
+        s->nlow.release();
+        s->nup.release();
+        s->uord.release();
     }
     chk(s->out_nlist.alloc(s->layout == PD_LAYOUT_SLOT ? (size_t)N * K : 0),
         "nlist export");
@@ -2282,11 +2581,6 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     rc = upload_tips(s);
     if (rc)
         return rc;
-    if (s->exact) {
-        rc = upload_inv(s);
-        if (rc)
-            return rc;
-    }
     s->has_buffers = true;
     s->retile_pending = false;
     adjoint_release(s);  // slot arrays were rebuilt
@@ -2381,6 +2675,7 @@ int pd_set_node_stiffness(pd_sim *s, const double *c)
     if (rc)
         return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    s->cnode_caller.release();
     if (!c) {
         s->cnode.release();
         s->h_cnode.clear();
@@ -2401,6 +2696,32 @@ int pd_set_node_stiffness(pd_sim *s, const double *c)
     return PD_OK;
 }
 
+int pd_set_node_stiffness_device(pd_sim *s, const double *c)
+{
+    int rc = check_sim(s);
+    if (rc)
+        return rc;
+    if (!c)
+        return fail(PD_E_ARG, "c is NULL (pd_set_node_stiffness clears)");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const size_t N = (size_t)s->n;
+    HIPCHK(s->cnode_caller.alloc(N));
+    HIPCHK(hipMemcpyAsync(s->cnode_caller.p, c, N * sizeof(double),
+                          hipMemcpyDeviceToDevice, s->stream));
+    s->h_cnode.clear();
+    s->has_cnode = true;
+    if (s->has_buffers) {
+        HIPCHK(s->cnode.alloc(N));
+        hipLaunchKernelGGL(k_perm, dim3((s->n + 255) / 256), dim3(256), 0,
+                           s->stream, s->n, 1, s->tnode.p, s->cnode_caller.p,
+                           s->cnode.p);
+        HIPCHK(hipGetLastError());
+        return refresh_records(s);  // synchronizes the stream
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return PD_OK;
+}
+
 int pd_set_break_mode(pd_sim *s, int tension_only)
 {
     int rc = check_sim(s);
@@ -2415,20 +2736,14 @@ int pd_set_exact_order(pd_sim *s, int on)
     int rc = check_sim(s);
     if (rc)
         return rc;
-    if (!on) {
-        s->exact = false;
-        s->inv.release();
-        return PD_OK;
-    }
-    if (s->layout != PD_LAYOUT_COMPACT || s->integrator != PD_EULER)
+    if (on && (s->layout != PD_LAYOUT_COMPACT || s->integrator != PD_EULER))
         return fail(PD_E_ARG, "the exact-order step is the Cython path's: "
                               "PD_LAYOUT_COMPACT and PD_EULER only");
-    if (s->has_buffers && !s->perm.empty()) {
-        rc = upload_inv(s);
-        if (rc)
-            return rc;
-    }
-    s->exact = true;
+    const bool was = s->exact;
+    s->exact = on != 0;
+    // the records carry V (exact) or 4kV (default): rewrite them
+    if (s->has_buffers && was != s->exact)
+        return refresh_records(s);
     return PD_OK;
 }
 
@@ -2646,7 +2961,12 @@ int adjoint_restore(pd_sim *s, int n0)
 
 }  // namespace
 
-int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
+namespace {
+
+// pd_adjoint_backward with dL_du and dL_dc in host memory (device false) or
+// in device memory (device true), both in the caller's node order.
+int adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc,
+                     bool device)
 {
     int rc = check_sim(s);
     if (rc)
@@ -2667,7 +2987,12 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
                           hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(s->fin_bforce.p, s->body_force.p, BB,
                           hipMemcpyDeviceToDevice, st));
-    {
+    if (device) {
+        hipLaunchKernelGGL(k_perm, dim3((s->n + 255) / 256), dim3(256), 0, st,
+                           s->n, 3, s->tnode.p, dL_du, s->lam[0].p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(s->gradc.p, 0, N * sizeof(double), st));
+    } else {
         const std::vector<double> v = to_internal(s->perm, dL_du, 3);
         HIPCHK(hipMemcpyAsync(s->lam[0].p, v.data(), BB,
                               hipMemcpyHostToDevice, st));
@@ -2756,9 +3081,53 @@ int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
     if (rc)
         return rc;
     s->step_counter = nsteps;
-    rc = download_caller(s, dL_dc, s->gradc.p, 1);
+    if (device) {
+        hipLaunchKernelGGL(k_unperm, dim3((s->n + 255) / 256), dim3(256), 0,
+                           st, s->n, 1, s->tnode.p, s->gradc.p, dL_dc);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        return PD_OK;
+    }
+    return download_caller(s, dL_dc, s->gradc.p, 1);
+}
+
+}  // namespace
+
+int pd_adjoint_backward(pd_sim *s, const double *dL_du, double *dL_dc)
+{
+    return adjoint_backward(s, dL_du, dL_dc, false);
+}
+
+int pd_adjoint_backward_device(pd_sim *s, const double *dL_du, double *dL_dc)
+{
+    return adjoint_backward(s, dL_du, dL_dc, true);
+}
+
+int pd_write_device(pd_sim *s, double *u, double *damage)
+{
+    int rc = check_sim(s);
     if (rc)
         return rc;
+    if (!s->has_buffers)
+        return fail(PD_E_STATE, "pd_create_buffers has not been called");
+    const int N = s->n;
+    const int blocks = (N + 255) / 256;
+    if (damage) {
+        hipLaunchKernelGGL(k_export, dim3(blocks), dim3(256), 0, s->stream,
+                           N, s->K, s->layout, 0, s->tnode.p, s->cnt.p,
+                           s->chunk0.p, s->stride.p, s->gidx.p, s->pos.p,
+                           s->family.p, s->n_neigh.p, s->out_nlist.p,
+                           s->out_nneigh.p, s->damage.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(damage, s->damage.p, (size_t)N * sizeof(double),
+                              hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (u) {
+        hipLaunchKernelGGL(k_unperm, dim3(blocks), dim3(256), 0, s->stream, N,
+                           3, s->tnode.p, s->u.p, u);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
     return PD_OK;
 }
 
@@ -2930,7 +3299,7 @@ void pd_destroy(pd_sim *s)
 
 int pd_bond_check(int device, int n, const double *ri, const double *rj,
                   const double *r0i, const double *r0j, double s0,
-                  int tension_only, double *l, double *l0, double *s,
+                  int flags, double *l, double *l0, double *s,
                   double *s_exact, int *broken)
 {
     if (n <= 0 || !ri || !rj || !r0i || !r0j || !l || !l0 || !s ||
@@ -2953,8 +3322,10 @@ int pd_bond_check(int device, int n, const double *ri, const double *rj,
     HIPCHK(d_se.alloc(N));
     HIPCHK(d_b.alloc(N));
     hipLaunchKernelGGL(k_bond_check, dim3((n + 255) / 256), dim3(256), 0, 0,
-                       n, d_ri.p, d_rj.p, d_r0i.p, d_r0j.p, s0,
-                       tension_only ? 0xffffffffu : 0x7fffffffu, d_l.p,
+                       n, (flags & PD_CHECK_OPENCL) ? 1 : 0, d_ri.p, d_rj.p,
+                       d_r0i.p, d_r0j.p, s0,
+                       (flags & PD_CHECK_TENSION) ? 0xffffffffu : 0x7fffffffu,
+                       d_l.p,
                        d_l0.p, d_s.p, d_se.p, d_b.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());

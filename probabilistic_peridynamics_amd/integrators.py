@@ -297,6 +297,32 @@ class Integrator(ABC):
         check(_lib.lib().pd_adjoint_backward(self._sim, dptr(g), dptr(out)))
         return out
 
+    def adjoint_backward_device(self, dL_du_ptr, dL_dc_ptr):
+        """:meth:`adjoint_backward` between device buffers of this
+        integrator's GPU, given by address: dL_du (nnodes, 3) f64 in,
+        dL_dc (nnodes,) f64 out, both in the caller's node order and
+        C-contiguous.  dL_du must be complete before the call; dL_dc is
+        complete on return."""
+        self._need_sim()
+        check(_lib.lib().pd_adjoint_backward_device(
+            self._sim, ctypes.c_void_p(int(dL_du_ptr)),
+            ctypes.c_void_p(int(dL_dc_ptr))))
+
+    def set_node_stiffness_device(self, c_ptr):
+        """:meth:`set_node_stiffness` from a device buffer (nnodes,) f64 of
+        this integrator's GPU, given by address; copied before return."""
+        self._need_sim()
+        check(_lib.lib().pd_set_node_stiffness_device(
+            self._sim, ctypes.c_void_p(int(c_ptr))))
+
+    def write_device(self, u_ptr=None, damage_ptr=None):
+        """u (nnodes, 3) and damage (nnodes,) f64 into device buffers of
+        this integrator's GPU, given by address (0/None to skip)."""
+        self._need_sim()
+        check(_lib.lib().pd_write_device(
+            self._sim, ctypes.c_void_p(int(u_ptr or 0)) if u_ptr else None,
+            ctypes.c_void_p(int(damage_ptr)) if damage_ptr else None))
+
     def adjoint_stop(self):
         self._need_sim()
         check(_lib.lib().pd_adjoint_stop(self._sim))
@@ -426,17 +452,19 @@ class Euler(Integrator):
     ``context`` is None so that :class:`~.model.Model` lays the neighbour
     list out as the Cython path does (model.py:580-582), and broken bonds are
     removed from it by the Cython compaction (peridynamics.pyx:102-156).
+    Each node's bond forces are summed in the order of
+    peridynamics.pyx:34-99, so force and u equal the Cython path's bitwise.
     """
 
     _kind = PD_EULER
     _layout = PD_LAYOUT_COMPACT
 
-    def __init__(self, dt, device=0, exact_order=False):
+    def __init__(self, dt, device=0, exact_order=True):
         """Create an :class:`Euler` integrator for HIP device ``device``.
 
-        ``exact_order=True`` sums each node's bond forces in the Cython
-        path's order (peridynamics.pyx:34-99), so that force and u equal it
-        bitwise: a slower verification mode (see :meth:`set_exact_order`).
+        ``exact_order=False`` sums each node's bonds in the fast step's own
+        order instead of the Cython path's: the same broken bonds, lists and
+        damage, force and u to rounding (see :meth:`set_exact_order`).
         """
         self.dt = dt
         self.context = None
@@ -478,13 +506,13 @@ class Euler(Integrator):
         super().build(nnodes, degrees_freedom, max_neighbours, coords, volume,
                       family, bc_types, bc_values, force_bc_types,
                       force_bc_values, None, None, None)
-        if self.exact_order:
-            self.set_exact_order(True)
+        self.set_exact_order(self.exact_order)
 
     def set_exact_order(self, on):
-        """Exact-order step on (True) or off: the default step sums a
-        node's bonds in its own order and matches the Cython path to
-        rounding; the exact-order step reproduces it bitwise."""
+        """Exact-order step on (True, the default) or off: the exact-order
+        step sums each node's bonds in the Cython path's order and
+        reproduces its force and u bitwise; the fast step sums them in its
+        own order and matches to rounding."""
         self._need_sim()
         check(_lib.lib().pd_set_exact_order(self._sim, int(bool(on))))
         self.exact_order = bool(on)

@@ -89,8 +89,7 @@ def test_dynamic_breaking_bitwise():
 
 def test_force_boundary_and_launch_shape_bitwise():
     """Force boundary conditions on the pulled end (added after the bond
-    sum, peridynamics.pyx:93-97) and another launch shape of the break
-    kernel: still bitwise."""
+    sum, peridynamics.pyx:93-97) and smaller tiles: still bitwise."""
     w = configs.bar_euler(layout="compact", dt=5e-15, pull=1e-5)
     rng = np.random.default_rng(2)
     end = w.coords[:, 0] > w.coords[:, 0].max() - 0.01
@@ -101,7 +100,7 @@ def test_force_boundary_and_launch_shape_bitwise():
     w.bc_types[end] = 0
     w.force_scale = 1.0
     integ, ref = _pair(w)
-    integ.set_launch(8, 64)
+    integ.set_launch(4, 64)
     z = np.zeros((w.nnodes, 3))
     integ.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,
                          w.critical_stretch, None, z, z, z, z, z,

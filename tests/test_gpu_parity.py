@@ -488,20 +488,47 @@ def _bar_sensitivity(w, steps=20):
     return pc.rel_l2(run(True), run(False))
 
 
-@pytest.mark.parametrize("layout", ["compact", "slot"])
-def test_bar_workload_full_length(layout):
+def _slot_order_sensitivity(w, steps):
+    """Relative L2 difference of the oracle's own u after ``steps`` steps
+    when every row's slots are summed in reverse order.  The OpenCL kernels
+    sum a node's slots by a tree (cl/peridynamics.cl:104-115), so any slot
+    order is an equally valid rounding of the reference's step: this is how
+    far its u is defined on the run."""
+    def run(nl):
+        ref = oracle.SlotStepper(0, w.dt, w.coords, w.volume, w.family,
+                                 w.bc_types, w.bc_values, w.force_bc_types,
+                                 w.force_bc_values, nl, w.bond_stiffness,
+                                 w.critical_stretch)
+        d, f = w.magnitudes(steps)
+        for s in range(steps):
+            ref(d[s], f[s])
+        return ref.u
+    return pc.rel_l2(run(np.ascontiguousarray(w.nlist[:, ::-1])),
+                     run(w.nlist))
+
+
+@pytest.mark.parametrize("layout,exact", [("compact", None),
+                                          ("compact", False),
+                                          ("slot", None)])
+def test_bar_workload_full_length(layout, exact):
     """The 2560-node pre-cracked bar exactly as specified (Euler, dt = 1e-5,
     1000 steps, pull 1e-8 per step).  The step is far past the stability
     limit: the pull throws the free end metres in two steps and a cascade
     breaks 27,328 bonds in step 3, after which the state stays put.  Lists,
-    counts and damage equal the oracle's exactly.  u cannot be held to
-    1e-10 here: bonds between nodes thrown metres get their stretch from
-    differences of large positions, so one ulp of u after step 2 (moved at
-    random) changes the oracle's own final u by ~7e-10.  u is held to 50
-    times that sensitivity, measured in the test."""
+    counts and damage equal the oracle's exactly in every mode.
+
+    u: the Cython-layout Euler as constructed by default, Euler(dt), sums in
+    peridynamics.pyx's order and equals the oracle bitwise.  Its fast order
+    (exact_order=False) sums each node's bonds in its own order: held to 50
+    times the run's measured sensitivity to one ulp of u after step 2.  The
+    slot layout (the OpenCL semantics, whose kernel sums a node's slots by a
+    tree) is held to 5 times the oracle's own difference under a reversed
+    slot order: the run amplifies last-bit differences of the first steps'
+    forces by ~1e8 per step, so u is defined only to that on this run."""
     w = configs.bar_euler(layout=layout)
     if layout == "compact":
-        integ = it.Euler(w.dt)
+        integ = (it.Euler(w.dt) if exact is None
+                 else it.Euler(w.dt, exact_order=exact))
         integ.build(w.nnodes, 3, w.max_neighbours, w.coords, w.volume,
                     w.family, w.bc_types, w.bc_values, w.force_bc_types,
                     w.force_bc_values, None, None, None)
@@ -533,9 +560,21 @@ def test_bar_workload_full_length(layout):
             np.testing.assert_array_equal(got["n_neigh"], rn)
             np.testing.assert_array_equal(got["damage"], rd)
     assert int(w.n_neigh.sum() - rn.sum()) == 27328
-    sens = _bar_sensitivity(configs.bar_euler(layout="slot"))
     err = pc.rel_l2(got["u"], ref.u)
-    print(f"bar, {layout}: u rel-L2 {err:.2e}, one-ulp sensitivity "
-          f"{sens:.2e}")
-    assert sens > 1e-10  # the run is this ill-conditioned
-    assert err < 50 * sens
+    if layout == "compact" and exact is None:
+        print(f"bar, compact, Euler(dt): u rel-L2 {err:.2e}")
+        assert integ.exact_order
+        np.testing.assert_array_equal(got["u"], ref.u)
+        np.testing.assert_array_equal(got["force"], ref.force)
+    elif layout == "compact":
+        sens = _bar_sensitivity(configs.bar_euler(layout="slot"))
+        print(f"bar, compact fast order: u rel-L2 {err:.2e}, one-ulp "
+              f"sensitivity {sens:.2e}")
+        assert sens > 1e-10  # the run is this ill-conditioned
+        assert err < 50 * sens
+    else:
+        sens = _slot_order_sensitivity(w, 1000)
+        print(f"bar, slot: u rel-L2 {err:.2e}, the oracle under a reversed "
+              f"slot order {sens:.2e}")
+        assert sens > 1e-4
+        assert err < 5 * sens
