@@ -271,6 +271,27 @@ int pd_bond_check(int device, int n, const double *ri, const double *rj,
                   int flags, double *l, double *l0, double *s,
                   double *s_exact, int *broken);
 
+/* Bond types of a body split by the plane coords[axis] = position, the
+ * bulk form of a per-bond is_bond_type predicate (model.py:709-780 calls it
+ * once per bond in Python): slot s < n_neigh[i] of row i gets `below` when
+ * both ends lie below the plane (x < position), `above` when both lie at or
+ * above it, `across` otherwise; the other slots get 0.  coords (N,3) f64,
+ * nlist (N,K) and n_neigh (N) i32, bond_types (N,K) i32 out.  device -1
+ * runs the same code on the host. */
+int pd_plane_bond_types(int device, int nnodes, int max_neighbours,
+                        const double *coords, const int *nlist,
+                        const int *n_neigh, int axis, double position,
+                        int below, int above, int across, int *bond_types);
+
+/* For every node, the index of the first of nboxes axis-aligned boxes
+ * [lo_b, hi_b) (lo (B,3), hi (B,3) f64; lo <= x < hi in every coordinate,
+ * bounds may be infinite) that contains it, -1 if none: the bulk form of
+ * per-node boundary-condition, density and tip predicates built from
+ * boxes (model.py:657-707, 1093-1212 call them once per node).  box (N)
+ * i32 out.  device -1 runs the same code on the host. */
+int pd_first_box(int device, int nnodes, const double *coords, int nboxes,
+                 const double *lo, const double *hi, int *box);
+
 /* Neighbour list of a point cloud (no counterpart kernel in the reference:
  * model.py:536-597 builds it on the host with a KD-tree).  Node j is a
  * neighbour of i (j != i) when ((dx*dx + dy*dy) + dz*dz) <= horizon*horizon,

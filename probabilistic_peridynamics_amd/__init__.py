@@ -13,12 +13,14 @@ from .model import (Model, initial_crack_helper, vectorised,
                     DimensionalityError, FamilyError, DamageModelError,
                     InvalidIntegrator)
 from .crack import LineCrack
+from .predicates import Boxes, PlaneBondTypes
 
 __all__ = [
     "Model", "initial_crack_helper", "Integrator", "Euler", "EulerHIP",
     "EulerCromerHIP", "VelocityVerletHIP", "EulerCL", "EulerCromerCL",
     "VelocityVerletCL", "HIPContext", "ContextError", "get_context",
     "DimensionalityError", "FamilyError", "DamageModelError",
-    "InvalidIntegrator", "LineCrack", "vectorised",
+    "InvalidIntegrator", "LineCrack", "vectorised", "Boxes",
+    "PlaneBondTypes",
 ]
 __version__ = "0.1.0"

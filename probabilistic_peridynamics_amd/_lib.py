@@ -95,6 +95,11 @@ SIGNATURES = {
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I, ctypes.c_int,
         ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
         ctypes.c_double, _I, ctypes.c_longlong, _LL]),
+    "pd_plane_bond_types": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _I, ctypes.c_int,
+        ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I]),
+    "pd_first_box": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]),
     "pd_create_crack": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _I, _I, _I]),
 }

@@ -2,8 +2,8 @@
 
 ``python -m probabilistic_peridynamics_amd.build`` compiles
 ``csrc/pd_hip.hip`` (time stepping), ``csrc/pd_nlist.hip`` (neighbour list),
-``csrc/pd_corrections.hip`` (stiffness corrections) and ``csrc/pd_crack.hip``
-(initial crack) into
+``csrc/pd_corrections.hip`` (stiffness corrections), ``csrc/pd_crack.hip``
+(initial crack) and ``csrc/pd_setup.hip`` (set-up predicates) into
 ``probabilistic_peridynamics_amd/libpd_hip.so``.  It
 needs hipcc but no GPU.
 
@@ -22,7 +22,8 @@ import sys
 
 HERE = pathlib.Path(__file__).resolve().parent
 SOURCES = [HERE / "csrc" / name for name in (
-    "pd_hip.hip", "pd_nlist.hip", "pd_corrections.hip", "pd_crack.hip")]
+    "pd_hip.hip", "pd_nlist.hip", "pd_corrections.hip", "pd_crack.hip",
+    "pd_setup.hip")]
 TARGET = HERE / "libpd_hip.so"
 ARCH = os.environ.get("PD_OFFLOAD_ARCH", "gfx950")
 

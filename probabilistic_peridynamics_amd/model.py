@@ -529,9 +529,19 @@ class Model(object):
             nlist, n_neigh = connectivity
             bond_types = np.zeros((self.nnodes, self.max_neighbours),
                                   dtype=np.intc)
-            if _is_vectorised(is_bond_type):
+            if callable(getattr(is_bond_type, "bond_types", None)):
+                # a built-in predicate (predicates.PlaneBondTypes): every
+                # bond at once in libpd_hip, with the reference's checks
+                bond_types = is_bond_type.bond_types(self.coords, nlist,
+                                                     n_neigh)
+                live = (np.arange(self.max_neighbours)[None, :]
+                        < np.asarray(n_neigh)[:, None])
+                self._check_bond_types(bond_types[live], nbond_types)
+            elif _is_vectorised(is_bond_type):
                 self._bond_types_vectorised(is_bond_type, nlist, n_neigh,
                                             nbond_types, bond_types)
+            if (_is_vectorised(is_bond_type)
+                    or callable(getattr(is_bond_type, "bond_types", None))):
                 if self.write_path is not None:
                     write_array(self.write_path, "bond_types", bond_types)
                 return bond_types
@@ -588,21 +598,26 @@ class Model(object):
                     "a vectorised is_bond_type must return an integer array "
                     "with one entry per bond (got {} of shape {} for {} "
                     "bonds)".format(bt.dtype, bt.shape, len(i)))
-            bad = np.nonzero((bt < 0) | (bt > nbond_types - 1))[0]
-            if bad.size:
-                q = bad[0]
-                if bt[q] < 0:
-                    raise ValueError(
-                        "is_bond_type must be a function that returns "
-                        "a *positive* int or 0 (got {})".format(bt[q]))
-                raise ValueError(
-                    "is_bond_type must be a function that returns "
-                    "a positive int or 0 which is *less* than "
-                    "nbond_types (nbond_types = {}, got {})".format(
-                        nbond_types, bt[q]))
+            self._check_bond_types(bt, nbond_types)
             sub = out[lo:hi]
             sub[live] = bt
             lo = hi
+
+    @staticmethod
+    def _check_bond_types(bt, nbond_types):
+        """The range checks of model.py:741-756 on many bond types."""
+        bad = np.nonzero((bt < 0) | (bt > nbond_types - 1))[0]
+        if bad.size:
+            q = bad[0]
+            if bt[q] < 0:
+                raise ValueError(
+                    "is_bond_type must be a function that returns "
+                    "a *positive* int or 0 (got {})".format(bt[q]))
+            raise ValueError(
+                "is_bond_type must be a function that returns "
+                "a positive int or 0 which is *less* than "
+                "nbond_types (nbond_types = {}, got {})".format(
+                    nbond_types, bt[q]))
 
     def _set_damage_model(self, bond_stiffness, critical_stretch):
         """Damage-model tables and plus_cs (model.py:941-1091).
