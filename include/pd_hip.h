@@ -230,6 +230,17 @@ int pd_set_tips(pd_sim *sim, int nentries, const int *node, const int *dof,
  * model sums use a fixed reduction order (not numpy's).  Synchronizes. */
 int pd_monitor(pd_sim *sim, double *model, double *tips);
 
+/* pd_monitor without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its partial sums
+ * and staged outputs live in a file-scope device array of the library
+ * (64 KiB per device, held until the process ends) instead of buffers
+ * allocated on first use; results equal pd_monitor's bitwise.  At most 1024
+ * tip groups (PD_E_LIMIT beyond).  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (from several threads or
+ * simulations at once) race; calls in sequence do not. */
+int pd_monitor_noalloc(pd_sim *sim, double *model, double *tips);
+
 /* Integrator.write (integrators.py:285-300): damage, then device -> host
  * copies.  Any pointer may be NULL to skip that array.  nlist (N,K) and
  * n_neigh (N) come back in the layout given to pd_build, reflecting every

@@ -65,6 +65,7 @@ SIGNATURES = {
     "pd_set_tips": (ctypes.c_int, [_SIM, ctypes.c_int, _I, _I, _I,
                                    ctypes.c_int]),
     "pd_monitor": (ctypes.c_int, [_SIM, _D, _D]),
+    "pd_monitor_noalloc": (ctypes.c_int, [_SIM, _D, _D]),
     "pd_get_regimes": (ctypes.c_int, [_SIM, _I]),
     "pd_live_bonds": (ctypes.c_int, [_SIM, _LL]),
     "pd_synchronize": (ctypes.c_int, [_SIM]),

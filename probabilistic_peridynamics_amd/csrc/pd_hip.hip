@@ -88,6 +88,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -1518,6 +1519,17 @@ __global__ __launch_bounds__(1024) void k_adj_tile(AdjArgs a)
 
 }  // namespace pdk
 
+// Device scratch of pd_monitor_noalloc: the per-block partial sums of
+// k_monitor_part, then the staged outputs of k_monitor_final (6 model sums
+// and 5 per tip group), for at most NOALLOC_MAX_GROUPS tip groups.  File
+// scope: every device the library runs on holds its own copy (64 KiB) until
+// the process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr int NOALLOC_MAX_GROUPS = 1024;
+constexpr size_t NOALLOC_DOUBLES = (size_t)pdk::MON_BLOCKS * pdk::MON_SUMS +
+                                   pdk::MON_SUMS + 5 * NOALLOC_MAX_GROUPS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -2666,6 +2678,84 @@ int pd_monitor(pd_sim *s, double *model, double *tips)
     std::copy(h.begin(), h.begin() + MON_SUMS, model);
     if (ngroups > 0)
         std::copy(h.begin() + MON_SUMS, h.end(), tips);
+    return PD_OK;
+}
+
+namespace {
+
+// The device address of `scratch` for each device, looked up once per
+// device with that device current.
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[NOALLOC_DEVICES] = {};
+
+int scratch_address(int device, double **out)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES)
+        return fail(PD_E_LIMIT, "pd_monitor_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch[device];
+    return PD_OK;
+}
+
+}  // namespace
+
+int pd_monitor_noalloc(pd_sim *s, double *model, double *tips)
+{
+    int rc = check_sim(s);  // makes the simulation's device current
+    if (rc)
+        return rc;
+    if (!s->has_buffers)
+        return fail(PD_E_STATE, "pd_create_buffers has not been called");
+    if (!model)
+        return fail(PD_E_ARG, "model (6 doubles) is required");
+    const int ngroups =
+        s->h_tip_off.empty() ? 0 : (int)s->h_tip_off.size() - 1;
+    if (ngroups > NOALLOC_MAX_GROUPS)
+        return fail(PD_E_LIMIT, "pd_monitor_noalloc: more than 1024 tip "
+                                "groups (use pd_monitor)");
+    if (ngroups > 0 && !tips)
+        return fail(PD_E_ARG, "tips (5 doubles per group) is required");
+    double *sp = nullptr;
+    rc = scratch_address(s->device, &sp);
+    if (rc)
+        return rc;
+    // every entry of both regions is written by the kernels below before it
+    // is read: nothing needs clearing between calls
+    double *part = sp;
+    double *staged = sp + (size_t)MON_BLOCKS * MON_SUMS;
+    const int N = s->n;
+    hipLaunchKernelGGL(k_export, dim3((N + 255) / 256), dim3(256), 0,
+                       s->stream, N, s->K, s->layout, 0, s->tnode.p,
+                       s->cnt.p, s->chunk0.p, s->stride.p, s->gidx.p,
+                       s->pos.p, s->family.p, s->n_neigh.p, s->out_nlist.p,
+                       s->out_nneigh.p, s->damage.p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_monitor_part, dim3(MON_BLOCKS), dim3(256), 0,
+                       s->stream, N, s->u.p, s->ud.p, s->udd.p, s->force.p,
+                       s->body_force.p, s->flags.p, s->vol.p, s->damage.p,
+                       part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s->stream,
+                       part, ngroups, s->tip_off.p, s->tip_q.p, s->u.p,
+                       s->ud.p, s->udd.p, s->force.p, s->body_force.p,
+                       s->flags.p, s->vol.p, staged);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(model, staged, MON_SUMS * sizeof(double),
+                          hipMemcpyDeviceToHost, s->stream));
+    if (ngroups > 0)
+        HIPCHK(hipMemcpyAsync(tips, staged + MON_SUMS,
+                              5 * (size_t)ngroups * sizeof(double),
+                              hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return PD_OK;
 }
 
