@@ -21,7 +21,8 @@ Prints one JSON line:
                  of the whole batch in one simulation, its steps captured in
                  one hipGraph, against the same samples run one after another
                  with plain per-step launches; launch shape from the
-                 autotuner run on the first 32 samples
+                 autotuner, each candidate timed as a graph run of the whole
+                 batch
   composite      the 1M-node composite trilinear Euler-Cromer bar (three
                  bond types, three regimes): step time, bond-updates/s and
                  the HBM fraction with its algorithmic bytes
@@ -159,7 +160,7 @@ def cpu_multicore(w, budget_s):
     }
 
 
-def uq_bench(nsamples, steps, tune_samples):
+def uq_bench(nsamples, steps, tune_steps):
     """Batched vs sequential sample-steps/s on the UQ batch."""
     from probabilistic_peridynamics_amd import autotune, batch, configs
     t0 = time.perf_counter()
@@ -167,15 +168,15 @@ def uq_bench(nsamples, steps, tune_samples):
     nodes = sum(w.nnodes for w in ws)
     log(f"[bench] uq: {len(ws)} samples, {nodes} nodes, built in "
         f"{time.perf_counter() - t0:.1f}s")
-    tb = batch.SampleBatch(ws[:tune_samples])
-    best, table = autotune.autotune_batch(tb, steps=10)
-    del tb
-    log(f"[bench] uq autotune: best {best} "
-        + " ".join(f"{c}:{ms:.2f}ms" for c, ms in table))
     t0 = time.perf_counter()
-    b = batch.SampleBatch(ws, lanes_per_node=best[0], tile_nodes=best[1])
-    log(f"[bench] uq batch set up in {time.perf_counter() - t0:.1f}s, "
-        f"{b.integrator.tile_stats()['ntiles']} tiles")
+    b = batch.SampleBatch(ws)
+    log(f"[bench] uq batch set up in {time.perf_counter() - t0:.1f}s")
+    # the whole batch, each candidate as a graph run (the timed form)
+    t0 = time.perf_counter()
+    best, table = autotune.autotune_batch(b, steps=tune_steps, graph=True)
+    log(f"[bench] uq autotune ({time.perf_counter() - t0:.1f}s): best {best} "
+        + " ".join(f"{c}:{ms:.2f}ms" for c, ms in table))
+    log(f"[bench] uq: {b.integrator.tile_stats()['ntiles']} tiles")
     ms_batched = b.run(steps, graph=True)
     live_end = b.integrator.live_bonds()
     del b
@@ -387,7 +388,9 @@ def main():
     ap.add_argument("--uq-samples", type=int, default=256,
                     help="samples of the batched UQ comparison (0 to skip)")
     ap.add_argument("--uq-steps", type=int, default=500)
-    ap.add_argument("--uq-tune-samples", type=int, default=32)
+    ap.add_argument("--uq-tune-steps", type=int, default=10,
+                    help="steps per autotuner candidate (graph runs of the "
+                    "whole batch)")
     ap.add_argument("--composite-steps", type=int, default=50,
                     help="steps of the composite workload (0 to skip)")
     ap.add_argument("--exact-steps", type=int, default=20,
@@ -450,7 +453,7 @@ def main():
     del integ
     uq = None
     if args.uq_samples > 0:
-        uq = uq_bench(args.uq_samples, args.uq_steps, args.uq_tune_samples)
+        uq = uq_bench(args.uq_samples, args.uq_steps, args.uq_tune_steps)
 
     comp = None
     if args.composite_steps > 0:

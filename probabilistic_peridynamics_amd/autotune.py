@@ -12,6 +12,15 @@ The step kernel has two shape parameters (``Integrator.set_launch``):
 The best pair depends on the mesh (bond counts, boundary fraction, number of
 tiles against the 256 CUs), so it is measured: each candidate is timed over
 a few steps from the same initial state, and the fastest is kept.
+
+PeriPy's OpenCL kernels give each node a workgroup and tile its neighbour
+list over the workgroup's work-items (cl/peridynamics.cl:134-262); a search
+over "nodes per workgroup" of 1-4 and "neighbour tiles" of 32-128 bonds
+belongs to that shape.  Here a workgroup holds a tile of 64-256 nodes whose
+halo it stages once, and a node's neighbour list is split over
+``lanes_per_node`` lanes in rounds of 4 * lanes_per_node bonds, so the
+searched sets are tile_nodes (nodes per workgroup) in {64, 128, 256} and
+lanes_per_node in {1, 2, 4, 8} (neighbour tiles of 4 to 32 bonds).
 """
 import itertools
 
@@ -49,14 +58,16 @@ def autotune(apply, time_steps, cands=None, repeats=1):
     return best, table
 
 
-def autotune_batch(batch, steps=10, cands=None):
-    """:func:`autotune` for a :class:`~.batch.SampleBatch`: each candidate is
-    timed over ``steps`` plain-launch steps from the initial state."""
+def autotune_batch(batch, steps=10, cands=None, graph=True):
+    """:func:`autotune` for a :class:`~.batch.SampleBatch`: each candidate
+    re-tiles the batch and is timed over ``steps`` steps from the initial
+    state, captured in one hipGraph (``graph``) as the batch's runs are, or
+    as plain launches."""
     def apply(lp, t):
         batch.set_launch(lp, t)
 
     def time_steps():
-        ms = batch.run(steps, graph=False)
+        ms = batch.run(steps, graph=graph)
         batch.integrator.synchronize()
         return ms
 

@@ -19,6 +19,9 @@ Workloads (sizes as named):
                Velocity-Verlet, damping 0, body force 1e4 N/m^3 in -z on
                the central strip |x - 1| < horizon, bottom nodes within a
                horizon of either end clamped in z.
+  beam_euler:  the beam_vv mesh and loading in the compact layout, no
+               densities, the Cython path's Euler with dt = 2e-16 (below
+               its explicit stability limit, about 8e-16).
   composite_ec: the beam_vv geometry centred on y = 0, bond types A
                (y < 0), B (y >= 0) and interface; trilinear law with
                breakpoints (0.002, 0.004, 0.008) and slopes (c, 0.6c, 0.1c),
@@ -213,13 +216,16 @@ def beam_vv(shape=(470, 47, 47), seed=1, layout="slot"):
 def beam_euler(shape=(470, 47, 47), seed=1):
     """The 1,038,230-node beam of beam_vv stepped by the Cython path's
     Euler integrator (compact layout): same mesh, supports and body-force
-    strip, no densities, a step (5e-15 s) inside the overdamped stability
-    limit."""
+    strip, no densities.  Explicit Euler (u += dt F) is stable for
+    dt < 2 / lambda_max of the bond stiffness operator; twice the largest
+    eigenvalue of a node's block sum_j c V_j e e^T / l over the beam's nodes
+    bounds lambda_max at 2.4e15, so dt < 8.3e-16, and the step is 2e-16 s
+    (5e-15 s, past that limit, shatters the beam within tens of steps)."""
     w = beam_vv(shape, seed, layout="compact")
     w.name = f"beam_euler_{w.nnodes}"
     w.integrator = "euler"
     w.densities = None
-    w.dt = 5e-15
+    w.dt = 2e-16
     w.extra["layout"] = "compact"
     return w
 

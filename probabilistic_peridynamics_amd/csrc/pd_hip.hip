@@ -58,9 +58,11 @@
 //   flags[N]   bit d: displacement BC on DOF d; bit 3+d: force BC on DOF d
 //   bcv, fbcv, dens (N,3) f64, read only for flagged DOFs / dynamic
 //              integrators
-//   nlist_ref (N,K) i32 + n_neigh: the reference-layout list of the compact
-//              (Cython) layout, in the caller's numbering, maintained by
-//              in-kernel compaction.
+//   rslot (N,K) u16 + n_neigh: the reference-layout list of the compact
+//              (Cython) layout, in the caller's numbering, as the slot index
+//              of each row position, maintained by in-kernel compaction;
+//              nlist_ref (N,K) i32 holds the caller's rows, rebuilt from
+//              rslot at export.
 //
 // Stretch arithmetic.  Compact layout (the Cython path): exactly
 // spatial.pyx's (dr = r_j - r_i with r = r0 + u, l = sqrt((dr0^2 + dr1^2) +
@@ -177,7 +179,8 @@ struct StepArgs {
     const double *k_tab, *s_tab, *pc_tab;
     const long long *chunk0;
     const int *stride;
-    int *nlist_ref, *n_neigh;
+    unsigned short *rslot;  // compact layout: row position -> slot
+    int *n_neigh;
     const double *cnode;  // node stiffness c (DM_FIELD)
     int *brk;  // if set: the step at which each slot's bond broke
     int step;  // index of this step (for brk)
@@ -236,70 +239,176 @@ __device__ __forceinline__ uint32_t ld_slot(const int *p)
                                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Is j a bond of node i that broke in this step?  Binary search of the
-// (ascending) keys of i's slots.
-__device__ bool broke_now(const int *gidx, long long c0, int stride, int cnt,
-                          int j)
-{
-    int lo = 0, hi = cnt - 1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const uint32_t e = ld_slot(&gidx[slot_of(c0, stride, mid)]);
-        const int key = (int)(e & KEYM);
-        if (key == j)
-            return (e & NEWB) != 0;
-        if (key < j)
-            lo = mid + 1;
-        else
-            hi = mid - 1;
-    }
-    return false;
-}
-
-// Reproduces, for row i of the reference-layout list, what
-// peripy/peridynamics.pyx:124-156 (break_bonds) does to that row in one pass:
+// --- compaction of the compact (Cython) layout's rows -----------------------
+//
+// rslot (N, K) u16, rows in the caller's numbering: position p < n_neigh[m]
+// of node m's reference-layout row holds the index of the slot (of m's
+// internal node) whose bond it lists, so the row's entries are those slots'
+// keys.  Positions from n_neigh[m] up to the row's count at set-up keep the
+// values that swap-with-last left there, as the reference's rows do; the
+// caller's row is rebuilt from rslot and gidx at export (k_export_rows).
+//
+// What peridynamics.pyx:124-156 (break_bonds) does to row i in one pass:
 // first every node k < i whose bond to i broke removes itself from i's row by
 // swap-with-last, in ascending k (the outer loop order of break_bonds), then
-// i's own sweep replaces each broken j > i by the last live entry without
-// advancing.  All of it depends only on row i and the set of bonds of i that
-// broke this step, which are the NEW slots of node i.
-__device__ __noinline__ void compact_row(int *gidx, long long c0, int stride,
-                                         int *nlist_ref, int K, int *n_neigh,
-                                         uint32_t i, int cnt)
+// i's own sweep replaces each broken j > i by the last entry without
+// advancing.  All of it depends only on row i and on which of node i's bonds
+// broke this step (its NEW slots).  Slots are sorted by key, so the lower
+// neighbours are slots [0, nlow) and the upper ones [nhi, cnt).
+
+__device__ __forceinline__ int slot_key(const int *gidx, long long c0,
+                                        int stride, int k)
 {
-    __threadfence();
-    int *R = nlist_ref + (size_t)i * K;
-    int n = n_neigh[i];
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = ld_slot(&gidx[slot_of(c0, stride, k)]);
-        const int j = (int)(e & KEYM);
-        if ((e & NEWB) && j < (int)i) {
-            for (int p = 0; p < n; ++p) {
-                if (R[p] == j) {
-                    R[p] = R[n - 1];
-                    --n;
-                    break;
-                }
-            }
+    return (int)(ld_slot(&gidx[slot_of(c0, stride, k)]) & KEYM);
+}
+
+// One node's row rs[0, n) in LDS, fl[k] = 1 for its slots that broke this
+// step; i is the node's caller id (slot keys are caller ids).  Applies
+// break_bonds' removals and returns the new count; nhi is set to the first
+// slot whose key exceeds i.
+__device__ __forceinline__ int compact_rows_seq(const int *gidx, long long c0,
+                                             int stride, int cnt, int nlow,
+                                             uint32_t i, unsigned short *rs,
+                                             const unsigned char *fl, int n,
+                                             int &nhi_out)
+{
+    int nhi = nlow;
+    while (nhi < cnt && slot_key(gidx, c0, stride, nhi) == (int)i)
+        ++nhi;
+    nhi_out = nhi;
+    for (int k = 0; k < nlow; ++k) {
+        if (!fl[k])
+            continue;
+        // slots [kl, kh] repeat k's key; the reference removes the first
+        // position holding that key
+        const int j = slot_key(gidx, c0, stride, k);
+        int kl = k, kh = k;
+        while (kl > 0 && slot_key(gidx, c0, stride, kl - 1) == j)
+            --kl;
+        while (kh + 1 < nlow && slot_key(gidx, c0, stride, kh + 1) == j)
+            ++kh;
+        int p = 0;
+        while (p < n && (rs[p] < kl || rs[p] > kh))
+            ++p;
+        if (p == n)
+            continue;
+        if (rs[p] != k) {
+            // a repeat: the position that held slot k keeps the key, with
+            // the slot that stays live
+            int p2 = p + 1;
+            while (p2 < n && rs[p2] != k)
+                ++p2;
+            if (p2 < n)
+                rs[p2] = rs[p];
         }
+        rs[p] = rs[n - 1];
+        --n;
     }
+    int rem = 0;
+    for (int k = nhi; k < cnt; ++k)
+        rem += fl[k];
     int p = 0;
-    while (p < n) {
-        const int j = R[p];
-        if (j > (int)i && broke_now(gidx, c0, stride, cnt, j)) {
-            R[p] = R[n - 1];
+    while (rem > 0 && p < n) {
+        const int k = rs[p];
+        if (k >= nhi && fl[k]) {
+            rs[p] = rs[n - 1];
             --n;
+            --rem;
         } else {
             ++p;
         }
     }
-    for (int k = 0; k < cnt; ++k) {
-        int *q = &gidx[slot_of(c0, stride, k)];
-        const uint32_t e = ld_slot(q);
-        if (e & NEWB)
-            *q = (int)(e & ~NEWB);
+    return n;
+}
+
+// LDS one node's row takes in compact_tile (u16 positions, u8 break flags);
+// the scratch also holds 64 bytes of counts.
+__host__ __device__ constexpr unsigned compact_row_bytes(int K)
+{
+    return (3u * (unsigned)K + 15u) & ~15u;
+}
+
+// Every thread of the block calls this after its bond loop, with nbroken the
+// node's count of bonds broken in this step (the same in all its lanes).  The
+// rows of the nodes that lost bonds are staged in `scratch` (`bytes` of LDS
+// the block no longer reads once every thread is here), compacted by one
+// lane each, and written back; with EXACT, the node's upper list (uord, nup)
+// is rebuilt in the new row order.  NEW flags are cleared.  Nodes take
+// scratch regions in groups when they do not all fit.  The first 64 bytes
+// hold each wave's count of such nodes, from which every node's region
+// follows without atomics.
+template <int LPN, bool EXACT>
+__device__ void compact_tile(const StepArgs &a, const TileInfo &ti,
+                             char *scratch, unsigned bytes, bool valid,
+                             int l, int sub, uint32_t i, int cnt, int nbroken)
+{
+    const bool mine = valid && nbroken > 0;
+    __syncthreads();  // every bond loop is done: the scratch is free
+    const unsigned long long bm = __ballot(mine && sub == 0);
+    const int lane = (int)(threadIdx.x & 63u), w = (int)(threadIdx.x >> 6);
+    int *ctr = reinterpret_cast<int *>(scratch);
+    if (lane == 0)
+        ctr[w] = __popcll(bm);
+    __syncthreads();
+    int total = 0, rank = 0;
+    for (int v = 0; v < (int)(blockDim.x >> 6); ++v) {
+        const int c = ctr[v];
+        total += c;
+        rank += v < w ? c : 0;
     }
-    n_neigh[i] = n;
+    if (total == 0)
+        return;  // the same in every thread
+    rank += __popcll(bm & ((1ull << lane) - 1ull));
+    rank = __shfl(rank, 0, LPN);  // the node's first lane holds its rank
+    const int K = a.K;
+    const unsigned nb = compact_row_bytes(K);
+    const int G = (int)((bytes - 64u) / nb);  // >= 1: checked by the host
+    const long long c0 = ti.chunk_off + l;
+    const uint32_t m = mine ? (uint32_t)a.tnode[i] : 0u;
+    unsigned short *R = a.rslot + (size_t)m * K;
+    for (int g0 = 0; g0 < total; g0 += G) {
+        const bool act = mine && rank >= g0 && rank < g0 + G;
+        unsigned short *rs = reinterpret_cast<unsigned short *>(
+            scratch + 64u + (unsigned)(act ? rank - g0 : 0) * nb);
+        unsigned char *fl = reinterpret_cast<unsigned char *>(rs + K);
+        int n = 0;
+        if (act) {
+            n = a.n_neigh[m];
+            for (int p = sub; p < n; p += LPN)
+                rs[p] = R[p];
+            for (int k = sub; k < cnt; k += LPN) {
+                int *q = &a.gidx[slot_of(c0, ti.nodes, k)];
+                const uint32_t e = ld_slot(q);
+                fl[k] = (e & NEWB) ? 1 : 0;
+                if (e & NEWB)
+                    *q = (int)(e & ~NEWB);
+            }
+        }
+        __syncthreads();
+        if (act && sub == 0) {
+            int nhi = 0;
+            // keys are caller ids: the node's own is m
+            const int n1 = compact_rows_seq(a.gidx, c0, ti.nodes, cnt,
+                                            a.nlow[i], m, rs, fl, n, nhi);
+            a.n_neigh[m] = n1;
+            if (EXACT) {
+                int c = 0;
+                for (int p = 0; p < n1; ++p) {
+                    const int k = rs[p];
+                    if (k >= nhi)
+                        a.uord[slot_of(c0, ti.nodes, c++)] =
+                            ((unsigned)k << 16) |
+                            a.loc[slot_of(c0, ti.nodes, k)];
+                }
+                a.nup[i] = c;
+            }
+        }
+        __syncthreads();
+        if (act)
+            for (int p = sub; p < n; p += LPN)
+                R[p] = rs[p];
+        __syncthreads();  // the next group reuses the scratch
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -385,7 +494,19 @@ __device__ __forceinline__ void bonds(const StepArgs &a, const char *ldsb,
         // k = (c_i + c_j)/2 for a field: s k/l V_j = (s/2)(0.5/l) w with
         // w = 2(c_i+c_j)V_j = c_i (2 V_j) + (2 c_j V_j), the staged pair
         w[q] = DM == DM_FIELD ? __builtin_fma(ci, p1.y, p3.y) : p1.y;
-        const bool fast = core_range(qq) & core_range(q0);
+        // sqrt_core is exact on [2^-767, inf).  With !(|s| < s0) as the
+        // break test, one compare covers both lengths: with q >= 2^-700, a
+        // q0 below 2^-767 (l0 < 2^-383.5, zero included) gives s > 2^33 or
+        // NaN and an infinite or NaN q0 gives NaN, so the bond breaks here
+        // as in the reference (unless s0 > 2^33); an infinite q gives NaN
+        // here and an infinite s there, both breaking.  q below 2^-700, or
+        // NaN, takes the exact path.  The tension test and the regime walk
+        // check both lengths.
+        bool fast;
+        if constexpr (DM != DM_NLIN && !TENS)
+            fast = qq >= 0x1p-700;
+        else
+            fast = core_range(qq) & core_range(q0);
         bool near;
         if (DM != DM_NLIN) {
             near = fabs((TENS ? sh[q] : fabs(sh[q])) - a.s0h) <= a.s0h_guard;
@@ -645,12 +766,15 @@ __global__ __launch_bounds__(1024) void k_step(StepArgs a, double dmag,
         fz += __shfl_xor(fz, off, LPN);
         nbroken += __shfl_xor(nbroken, off, LPN);
     }
+    // the compact layout (the Cython path, launched with !CLA only): once
+    // the halo is no longer read, its LDS holds the rows being compacted
+    if constexpr (!CLA) {
+        if (a.layout == PD_LAYOUT_COMPACT)
+            compact_tile<LPN, false>(a, ti, ldsb, LDS_BYTES, valid, l, sub,
+                                     i, cnt, nbroken);
+    }
     if (!valid)
         return;
-
-    if (sub == 0 && a.layout == PD_LAYOUT_COMPACT && nbroken > 0)
-        compact_row(a.gidx, ti.chunk_off + l, ti.nodes, a.nlist_ref, a.K,
-                    a.n_neigh, (uint32_t)a.tnode[i], cnt);
 
     // the node's lanes share its three components (after the xor reduction
     // every lane holds the sums), so the per-node loads and stores coalesce
@@ -842,6 +966,26 @@ __global__ void k_export(int n, int K, int layout, int rows, const int *tnode,
     damage[i] = (f - (double)live) / f;
 }
 
+// The compact layout's reference-layout rows in the caller's numbering: the
+// positions below the row's count at set-up list their slots' keys (live
+// ones first, then what swap-with-last left behind); the others keep the
+// caller's values (uploaded into nlist_ref at set-up).
+__global__ void k_export_rows(int n, int K, const int *tnode, const int *cnt,
+                              const long long *chunk0, const int *stride,
+                              const int *gidx, const unsigned short *rslot,
+                              int *nlist_ref)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const size_t row = (size_t)tnode[p] * K;
+    const int c = cnt[p];
+    for (int q = 0; q < c; ++q)
+        nlist_ref[row + q] =
+            (int)((uint32_t)gidx[slot_of(chunk0[p], stride[p],
+                                         rslot[row + q])] & KEYM);
+}
+
 // Monitored sums of a write (model.py:1320-1360).  Pass 1: block b sums a
 // fixed strided share of the DOFs -- u, ud, udd, force*V, body_force*V --
 // and of the nodes' damage, then reduces its 256 lanes by a fixed tree, so
@@ -977,7 +1121,7 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
 // A node's fold list is its lower slots in ascending order (slot order:
 // rows are sorted by caller id; a dead slot adds nothing), then its upper
 // neighbours in current row order (uord: slot index << 16 | halo loc,
-// rebuilt by compact_row after every break of the node).  The node's four
+// rebuilt by compact_tile after every break of the node).  The node's four
 // lanes take four consecutive positions per round: each evaluates its bond
 // with the reference's operations -- bitwise square roots, IEEE divisions,
 // no contraction -- decides the break from that exact stretch
@@ -995,51 +1139,40 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
 constexpr int EXACT_NODES = 128;
 constexpr int EXACT_CAP = (int)((LDS_BYTES - EXACT_NODES * 96u) / 64u);
 
-// Slot index of caller id j among node i's slots (keys ascending).
-__device__ int slot_index(const int *gidx, long long c0, int stride, int cnt,
-                          int j)
-{
-    int lo = 0, hi = cnt - 1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int key =
-            (int)(ld_slot(&gidx[slot_of(c0, stride, mid)]) & KEYM);
-        if (key == j)
-            return mid;
-        if (key < j)
-            lo = mid + 1;
-        else
-            hi = mid - 1;
-    }
-    return -1;
-}
-
-// The upper list of node p (caller id i) in the order of its current
-// reference-layout row.
-__device__ __noinline__ void rebuild_uord(const int *gidx,
-                                          const unsigned short *loc,
-                                          unsigned *uord, int *nup,
-                                          long long c0, int stride,
-                                          const int *row, int n, uint32_t i,
-                                          int cnt, int p)
-{
-    int c = 0;
-    for (int q = 0; q < n; ++q) {
-        const int j = row[q];
-        if (j > (int)i) {
-            const int k = slot_index(gidx, c0, stride, cnt, j);
-            uord[slot_of(c0, stride, c++)] =
-                ((unsigned)k << 16) | loc[slot_of(c0, stride, k)];
-        }
-    }
-    nup[p] = c;
-}
-
 // The exact bond evaluation: the term's components and the stretch.
 struct ExactTerm {
     double t0, t1, t2, s;
 };
 
+// a / b rounded to nearest by the compiler's f64 division sequence (rcp, two
+// Newton steps, the quotient and one residual correction) without its
+// v_div_scale / v_div_fixup steps.  Those change nothing -- the result is
+// the IEEE quotient -- when a is zero, or when a, b and a/b are normal, a >=
+// 2^-969, 1/b is normal and a's exponent exceeds b's by less than 768.
+__device__ __forceinline__ double div_core(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
+
+// The exact bond evaluation.  With q and q0 at least 2^-700, l and l0 come
+// from sqrt_core (exact there, and l, l0 lie in [2^-350, 2^512]).  With the
+// stiffness k in [2^-100, 2^100] and s0 below 2^300 (SAFE, checked by the
+// host) both quotients come from div_core: s = (l - l0)/l0 is 0 or at least
+// 2^-54 in magnitude (l - l0 is 0 or at least an ulp of the shorter), and for a
+// bond that does not break it is below s0, so every operand and quotient is
+// normal, numerators exceed 2^-969 and no exponent gap reaches 768; a bond
+// that breaks only needs its stretch on the right side of s0, which a huge
+// (or NaN: l0 from a q0 outside the range) value is, as the reference's
+// infinite or NaN one is.  The rest take the library's square roots and
+// divisions.
+template <bool SAFE>
 __device__ __forceinline__ ExactTerm exact_term(const char *ldsb, unsigned e,
                                                 const double4 &r0m,
                                                 const double4 &rm, double k)
@@ -1055,18 +1188,25 @@ __device__ __forceinline__ ExactTerm exact_term(const char *ldsb, unsigned e,
     const double q = (dx * dx + dy * dy) + dz * dz;
     const double ex = p0.x - r0m.x, ey = p0.y - r0m.y, ez = p1.x - r0m.z;
     const double q0 = (ex * ex + ey * ey) + ez * ez;
+    const bool fast = (q >= 0x1p-700) & (q0 >= 0x1p-700);
     double l, l0, h;
     sqrt_core(q, l, h);
     sqrt_core(q0, l0, h);
-    if (!(core_range(q) & core_range(q0))) {
-        l = sqrt(q);
-        l0 = sqrt(q0);
-    }
     ExactTerm t;
+    double fn;
     // spatial.pyx:87-97, peridynamics.pyx:75-89
-    t.s = (l - l0) / l0;
-    double fn = t.s * k;
-    fn = fn / l;
+    if (SAFE && fast) {
+        t.s = div_core(l - l0, l0);
+        fn = div_core(t.s * k, l);
+    } else {
+        if (!fast) {
+            l = sqrt(q);
+            l0 = sqrt(q0);
+        }
+        t.s = (l - l0) / l0;
+        fn = t.s * k;
+        fn = fn / l;
+    }
     const double v = p1.y;  // V_j
     t.t0 = (fn * dx) * v;
     t.t1 = (fn * dy) * v;
@@ -1096,6 +1236,7 @@ __device__ __forceinline__ void window_done()
 // One workgroup per tile of at most EXACT_NODES nodes, four lanes per node
 // (block 4 * tile_nodes), on k_step's tiles and halo staging.  Records hold
 // (r0, V | r0 + u, 0).
+template <bool SAFE>
 __global__ __launch_bounds__(512) void k_exact_step(StepArgs a, double dmag,
                                                     double fmag)
 {
@@ -1158,21 +1299,27 @@ __global__ __launch_bounds__(512) void k_exact_step(StepArgs a, double dmag,
     int nbroken = 0;
     if (valid) {
         const int total = nlow + nup;
+        // fold position -> slot index << 16 | halo loc (lower positions are
+        // the slots themselves, upper ones come from uord), requested two
+        // rounds ahead of their use
+        auto entry = [&](int pos) -> unsigned {
+            if (pos < nlow)
+                return ((unsigned)pos << 16) |
+                       a.loc[slot_of(c0, ti.nodes, pos)];
+            if (pos < total)
+                return a.uord[slot_of(c0, ti.nodes, pos - nlow)];
+            return LOC_DEAD;
+        };
+        unsigned ahead0 = entry(sub), ahead1 = entry(sub + LPN);
         for (int r = 0; r < total; r += LPN) {
-            const int pos = r + sub;
-            unsigned e = LOC_DEAD;
-            size_t slot = 0;
-            if (pos < nlow) {
-                slot = slot_of(c0, ti.nodes, pos);
-                e = a.loc[slot];
-            } else if (pos < total) {
-                const unsigned u = a.uord[slot_of(c0, ti.nodes, pos - nlow)];
-                slot = slot_of(c0, ti.nodes, (int)(u >> 16));
-                e = u & 0xffffu;
-            }
+            const unsigned cur = ahead0;
+            ahead0 = ahead1;
+            ahead1 = entry(r + 2 * LPN + sub);
+            const unsigned e = cur & 0xffffu;
+            const size_t slot = slot_of(c0, ti.nodes, (int)(cur >> 16));
             ExactTerm t = {0.0, 0.0, 0.0, 0.0};
             if (e != LOC_DEAD) {
-                t = exact_term(ldsb, e, r0m, rm, a.kx);
+                t = exact_term<SAFE>(ldsb, e, r0m, rm, a.kx);
                 const double as = sign_mask(t.s, a.absmask);
                 if (!(as < s0)) {  // peridynamics.pyx:133
                     a.loc[slot] = LOC_DEAD;
@@ -1198,19 +1345,14 @@ __global__ __launch_bounds__(512) void k_exact_step(StepArgs a, double dmag,
 #pragma unroll
     for (int off = LPN / 2; off > 0; off >>= 1)
         nbroken += __shfl_xor(nbroken, off, LPN);
+    // the fold windows are free: they hold the rows being compacted, and
+    // the upper lists of those nodes are rebuilt in the new row order
+    compact_tile<LPN, true>(a, ti, ldsb + 4 * PB, EXACT_NODES * 96u, valid, l,
+                            sub, i, cnt, nbroken);
     if (!valid)
         return;
     if (nbroken > 0) {
-        // compaction, then the upper part again in the new row order
-        const uint32_t m = (uint32_t)a.tnode[i];
-        if (sub == 0) {
-            compact_row(a.gidx, c0, ti.nodes, a.nlist_ref, a.K, a.n_neigh, m,
-                        cnt);
-            rebuild_uord(a.gidx, a.loc, a.uord, a.nup, c0, ti.nodes,
-                         a.nlist_ref + (size_t)m * a.K, a.n_neigh[m], m, cnt,
-                         (int)i);
-        }
-        __threadfence();
+        // the upper part again, in the new row order
         const int nup2 = ld_slot(&a.nup[i]);
         F = Flow;
         for (int r = 0; r < nup2; r += LPN) {
@@ -1219,7 +1361,7 @@ __global__ __launch_bounds__(512) void k_exact_step(StepArgs a, double dmag,
             if (pos < nup2) {
                 const unsigned u = ld_slot(reinterpret_cast<const int *>(
                     &a.uord[slot_of(c0, ti.nodes, pos)]));
-                t = exact_term(ldsb, u & 0xffffu, r0m, rm, a.kx);
+                t = exact_term<SAFE>(ldsb, u & 0xffffu, r0m, rm, a.kx);
             }
             window_put(win, sub, t);
             if (sub < 3) {
@@ -1610,7 +1752,7 @@ struct pd_sim {
     DevBuf<int> flags, family, cnt, tnode, halo, gidx, stride, nlist_ref,
         n_neigh, out_nlist, out_nneigh;
     DevBuf<long long> chunk0;
-    DevBuf<unsigned short> loc, pos, state;
+    DevBuf<unsigned short> loc, pos, state, rslot;
     DevBuf<unsigned long long> counter;
 
     // adjoint recording (Euler, node field): break steps, checkpoints
@@ -1757,7 +1899,7 @@ StepArgs make_args(pd_sim *s)
     a.pc_tab = s->pc_tab.p;
     a.chunk0 = s->chunk0.p;
     a.stride = s->stride.p;
-    a.nlist_ref = s->nlist_ref.p;
+    a.rslot = s->rslot.p;
     a.n_neigh = s->n_neigh.p;
     a.cnode = s->cnode.p;
     a.brk = s->adj_recording ? s->brk.p : nullptr;
@@ -1927,9 +2069,18 @@ int launch_step(pd_sim *s, double dmag, double fmag)
     const StepArgs a = make_args(s);
     hipError_t e;
     if (s->exact) {
-        hipLaunchKernelGGL(k_exact_step, dim3(XCDS * a.tiles_per_xcd),
-                           dim3(4 * s->tile_nodes), 0, s->stream, a, dmag,
-                           fmag);
+        // the stiffness range in which div_core is the IEEE division
+        const double ak = std::fabs(s->k);
+        if (ak >= 0x1p-100 && ak <= 0x1p100 && s->s0 < 0x1p300)
+            hipLaunchKernelGGL(k_exact_step<true>,
+                               dim3(XCDS * a.tiles_per_xcd),
+                               dim3(4 * s->tile_nodes), 0, s->stream, a, dmag,
+                               fmag);
+        else
+            hipLaunchKernelGGL(k_exact_step<false>,
+                               dim3(XCDS * a.tiles_per_xcd),
+                               dim3(4 * s->tile_nodes), 0, s->stream, a, dmag,
+                               fmag);
         e = hipGetLastError();
     } else {
         switch (dm_of(s)) {
@@ -2096,6 +2247,10 @@ int pd_build(pd_sim **out, int device, int integrator, int layout, double dt,
         return fail(PD_E_LIMIT, "nnodes must be in 1..2^26");
     if (max_neighbours <= 0 || max_neighbours > 65535)
         return fail(PD_E_LIMIT, "max_neighbours must be in 1..65535");
+    if (layout == PD_LAYOUT_COMPACT &&
+        compact_row_bytes(max_neighbours) + 64u > LDS_BYTES)
+        return fail(PD_E_LIMIT, "the compact layout steps at most 27285 "
+                                "neighbours per node");
     if (!coords || !volume || !family || !bc_types || !bc_values ||
         !force_bc_types || !force_bc_values)
         return fail(PD_E_ARG, "a required build array is NULL");
@@ -2315,16 +2470,26 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
               1;
     const std::vector<int> order = morton_order(
         N, s->h_coords.data(), s->h_group.empty() ? nullptr : s->h_group.data());
-    std::vector<int> stamp(N, -1);
+    std::vector<int> stamp(N, -1), probe(N, -1);
     std::vector<TileInfo> tiles;
     std::vector<int> tnode;
     tnode.reserve(N);
     {
-        int cur_nodes = 0, cur_halo = 0, tid = 0;
+        int cur_nodes = 0, cur_halo = 0, tid = 0, nprobe = 0;
+        // distinct nodes of i's neighbourhood not yet in tile t (a row may
+        // list a neighbour more than once)
         auto count_new = [&](int i, int t) {
-            int c = stamp[i] != t;
+            ++nprobe;
+            int c = 0;
+            auto visit = [&](int j) {
+                if (stamp[j] != t && probe[j] != nprobe) {
+                    probe[j] = nprobe;
+                    ++c;
+                }
+            };
+            visit(i);
             for (long long q = rstart[i]; q < rstart[i + 1]; ++q)
-                c += stamp[rj[q]] != t;
+                visit(rj[q]);
             return c;
         };
         for (int q = 0; q < N; ++q) {
@@ -2404,6 +2569,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     const bool compact = s->layout == PD_LAYOUT_COMPACT;
     std::vector<int> nlow_h(compact ? N : 0), nup_h(compact ? N : 0);
     std::vector<unsigned> uord_h(compact ? S : 0, 0u);
+    std::vector<unsigned short> rslot_h(compact ? (size_t)N * K : 0, 0);
     std::atomic<int> halo_mismatch{0};
     run_parallel(ntiles, [&](int lo, int hi) {
         std::vector<int> lmap(N, -1);  // internal id -> halo position
@@ -2446,8 +2612,11 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                             (s->h_btypes[src] << 8) | (regimes ? regimes[src] : 0));
                 }
                 if (compact) {
-                    // the exact step's lists: lower slots, and the upper
-                    // neighbours in the row's order
+                    // row position -> slot, then the exact step's lists:
+                    // lower slots, and the upper neighbours in row order
+                    for (int k = 0; k < cnt[i]; ++k)
+                        rslot_h[(size_t)i * K + rp[rstart[i] + k]] =
+                            (unsigned short)k;
                     const int p = ti.node_off + l;
                     const int *rb = rj.data() + rstart[i];
                     nlow_h[p] = (int)(std::lower_bound(rb, rb + cnt[i], i) - rb);
@@ -2543,12 +2712,14 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
         s->state.release();
     if (s->layout == PD_LAYOUT_COMPACT) {
         chk(upload(s->nlist_ref, nlist, (size_t)N * K), "nlist");
+        chk(upload(s->rslot, rslot_h.data(), rslot_h.size()), "nlist");
         chk(upload(s->n_neigh, n_neigh, (size_t)N), "n_neigh");
         chk(upload(s->nlow, nlow_h.data(), nlow_h.size()), "exact lists");
         chk(upload(s->nup, nup_h.data(), nup_h.size()), "exact lists");
         chk(upload(s->uord, uord_h.data(), uord_h.size()), "exact lists");
     } else {
         s->nlist_ref.release();
+        s->rslot.release();
         s->n_neigh.release();
         s->nlow.release();
         s->nup.release();
@@ -2829,6 +3000,9 @@ int pd_set_exact_order(pd_sim *s, int on)
     if (on && (s->layout != PD_LAYOUT_COMPACT || s->integrator != PD_EULER))
         return fail(PD_E_ARG, "the exact-order step is the Cython path's: "
                               "PD_LAYOUT_COMPACT and PD_EULER only");
+    if (on && compact_row_bytes(s->K) + 16u > EXACT_NODES * 96u)
+        return fail(PD_E_LIMIT, "the exact-order step takes at most 4090 "
+                                "neighbours per node");
     const bool was = s->exact;
     s->exact = on != 0;
     // the records carry V (exact) or 4kV (default): rewrite them
@@ -3248,6 +3422,13 @@ int pd_write(pd_sim *s, double *u, double *ud, double *udd, double *force,
                        s->n_neigh.p, s->out_nlist.p, s->out_nneigh.p,
                        s->damage.p);
     HIPCHK(hipGetLastError());
+    if (nlist && s->layout == PD_LAYOUT_COMPACT) {
+        hipLaunchKernelGGL(k_export_rows, dim3(blocks), dim3(256), 0,
+                           s->stream, N, s->K, s->tnode.p, s->cnt.p,
+                           s->chunk0.p, s->stride.p, s->gidx.p, s->rslot.p,
+                           s->nlist_ref.p);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     const struct {
         double *dst;

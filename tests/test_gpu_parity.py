@@ -230,6 +230,63 @@ def test_dynamic_breaking_compact_layout():
     assert integ.live_bonds() == int(ref.n_neigh.sum())
 
 
+def _with_repeats(w, nrep=300, seed=7):
+    """The compact-layout bar with ``nrep`` bonds listed twice in both of
+    their rows (the second copy appended), family counting the copies."""
+    rng = np.random.default_rng(seed)
+    n, k = w.nlist.shape
+    nl = np.zeros((n, k + 4), dtype=w.nlist.dtype)
+    nl[:, :k] = w.nlist
+    nn = w.n_neigh.copy()
+    done = 0
+    for i in rng.permutation(n):
+        if done == nrep:
+            break
+        if nn[i] == 0 or nn[i] >= k + 4:
+            continue
+        j = int(nl[i, rng.integers(nn[i])])
+        if nn[j] >= k + 4 or i in nl[j, w.n_neigh[j]:nn[j]] or \
+                j in nl[i, w.n_neigh[i]:nn[i]]:
+            continue
+        nl[i, nn[i]] = j
+        nn[i] += 1
+        nl[j, nn[j]] = i
+        nn[j] += 1
+        done += 1
+    w.nlist, w.n_neigh, w.family = nl, nn, nn.copy()
+    return w
+
+
+def test_dynamic_breaking_compact_layout_with_repeats():
+    """Rows that list a bond twice: the reference's compaction removes the
+    first position holding a broken bond's key each time it breaks; lists,
+    counts and damage equal the oracle's at every checkpoint."""
+    w = _with_repeats(_bar("compact"))
+    integ = it.Euler(w.dt, exact_order=False)
+    integ.build(w.nnodes, 3, w.max_neighbours, w.coords, w.volume, w.family,
+                w.bc_types, w.bc_values, w.force_bc_types,
+                w.force_bc_values, None, None, None)
+    z = np.zeros((w.nnodes, 3))
+    integ.create_buffers(w.nlist, w.n_neigh, w.bond_stiffness,
+                         w.critical_stretch, None, z, z, z, z, z,
+                         np.zeros(w.nnodes), None, 1, 1)
+    ref = oracle.CythonEuler(w.dt, w.coords, w.volume, w.family, w.bc_types,
+                             w.bc_values, w.force_bc_types,
+                             w.force_bc_values, w.nlist, w.n_neigh,
+                             w.bond_stiffness, w.critical_stretch)
+    d, f = w.magnitudes(60)
+    for s in range(60):
+        integ(d[s], f[s])
+        ref(d[s], f[s])
+        if s % 15 == 14:
+            got = _state(integ)
+            np.testing.assert_array_equal(got["nlist"], ref.nlist)
+            np.testing.assert_array_equal(got["n_neigh"], ref.n_neigh)
+            np.testing.assert_array_equal(got["damage"], ref.damage())
+            assert pc.rel_l2(got["u"], ref.u) < PARITY_RTOL
+    assert int(w.n_neigh.sum() - ref.n_neigh.sum()) > 4000
+
+
 @pytest.mark.parametrize("tension_only", [False, True])
 def test_dynamic_breaking_slot_layout(tension_only):
     w = _bar("slot", pull=1e-5)
