@@ -2149,6 +2149,85 @@ uint64_t spread3(uint64_t v)
     return v;
 }
 
+// LDS entries of a tile's halo, chosen so that the records the bond loop
+// reads together fall into different banks.  One ds_read_b128 over 16
+// consecutive lanes reads a 16-byte piece of each lane's neighbour record;
+// records whose entries are equal modulo HALO_BANK_GROUPS share banks, and
+// distinct records in one bank are read one after another.  rows[l] lists
+// the tile's node l's neighbours as halo indices in slot order; lane
+// t = l * lpn + sub takes the chunks of four slots sub, sub + lpn, ...  Read
+// groups are (16 lanes, round, bond of the chunk).  Greedy: records in the
+// most groups first, each to the residue least loaded over its groups among
+// the entries left (entry 0 is the zero record).  Returns entry - 1 for each
+// halo index; the order changes no result.
+constexpr int HALO_BANK_GROUPS = 16;
+
+std::vector<int> spread_halo(const std::vector<std::vector<int>> &rows,
+                             int lpn, int H)
+{
+    constexpr int NB = HALO_BANK_GROUPS;
+    std::vector<std::vector<int>> memb(H);
+    std::vector<int> seen(H, -1);
+    const int lanes = (int)rows.size() * lpn;
+    int ngroups = 0;
+    for (int q0 = 0; q0 < lanes; q0 += 16) {
+        const int q1 = std::min(q0 + 16, lanes);
+        int rounds = 0;
+        for (int t = q0; t < q1; ++t) {
+            const int nch = ((int)rows[t / lpn].size() + 3) / 4;
+            rounds = std::max(rounds, (nch - t % lpn + lpn - 1) / lpn);
+        }
+        for (int k = 0; k < rounds; ++k)
+            for (int b = 0; b < 4; ++b) {
+                const int g = ngroups++;
+                for (int t = q0; t < q1; ++t) {
+                    const std::vector<int> &row = rows[t / lpn];
+                    const int p = 4 * (t % lpn + lpn * k) + b;
+                    if (p < (int)row.size() && seen[row[p]] != g) {
+                        seen[row[p]] = g;
+                        memb[row[p]].push_back(g);
+                    }
+                }
+            }
+    }
+    std::vector<int> avail(NB, 0);
+    for (int e = 1; e <= H; ++e)
+        ++avail[e % NB];
+    std::vector<int> order(H);
+    for (int h = 0; h < H; ++h)
+        order[h] = h;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return memb[x].size() > memb[y].size();
+    });
+    std::vector<unsigned short> load((size_t)ngroups * NB, 0);
+    std::vector<int> res(H);
+    int score[NB];
+    for (int h : order) {
+        for (int r = 0; r < NB; ++r)
+            score[r] = 0;
+        for (int g : memb[h])
+            for (int r = 0; r < NB; ++r)
+                score[r] += load[(size_t)g * NB + r];
+        int best = -1;
+        for (int r = 0; r < NB; ++r)
+            if (avail[r] > 0 && (best < 0 || score[r] < score[best]))
+                best = r;
+        res[h] = best;
+        --avail[best];
+        for (int g : memb[h])
+            ++load[(size_t)g * NB + best];
+    }
+    int next[NB];
+    for (int r = 0; r < NB; ++r)
+        next[r] = r == 0 ? NB : r;
+    std::vector<int> pos(H);
+    for (int h = 0; h < H; ++h) {
+        pos[h] = next[res[h]] - 1;
+        next[res[h]] += NB;
+    }
+    return pos;
+}
+
 // Node order for tiling: Morton order of the initial coordinates on a grid
 // whose cell holds about one node.
 // Node order for tiling: by group (when given), then by Morton code of the
@@ -2574,6 +2653,7 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
     run_parallel(ntiles, [&](int lo, int hi) {
         std::vector<int> lmap(N, -1);  // internal id -> halo position
         std::vector<int> hl;
+        std::vector<std::vector<int>> rows;
         for (int t = lo; t < hi; ++t) {
             const TileInfo &ti = tiles[t];
             hl.clear();
@@ -2589,9 +2669,20 @@ int pd_create_buffers(pd_sim *s, const int *nlist, const int *n_neigh,
                 halo_mismatch = 1;
                 continue;
             }
-            for (size_t h = 0; h < hl.size(); ++h) {
+            for (size_t h = 0; h < hl.size(); ++h)
                 lmap[hl[h]] = (int)h;
-                halo[ti.halo_off + h] = hl[h];
+            rows.resize(ti.nodes);
+            for (int l = 0; l < ti.nodes; ++l) {
+                const int i = tnode[ti.node_off + l];
+                rows[l].clear();
+                for (int k = 0; k < cnt[i]; ++k)
+                    rows[l].push_back(lmap[inv[rj[rstart[i] + k]]]);
+            }
+            const std::vector<int> hpos =
+                spread_halo(rows, s->lpn, (int)hl.size());
+            for (size_t h = 0; h < hl.size(); ++h) {
+                lmap[hl[h]] = hpos[h];
+                halo[ti.halo_off + hpos[h]] = hl[h];
             }
             for (int l = 0; l < ti.nodes; ++l) {
                 const int i = tnode[ti.node_off + l];
