@@ -20,9 +20,12 @@ Prints one JSON line:
                  stiffness fields, Euler dt 1e-5, 500 steps): sample-steps/s
                  of the whole batch in one simulation, its steps captured in
                  one hipGraph, against the same samples run one after another
-                 with plain per-step launches; launch shape from the
-                 autotuner, each candidate timed as a graph run of the whole
-                 batch
+                 with plain per-step launches (GPU time of back-to-back
+                 launches); the batch's launch shape from the autotuner, each
+                 candidate timed as a graph run of the whole batch, the
+                 sequential runs' from their own fastest shape (timed on 16
+                 samples spread over the sizes), with the drop-in's default
+                 4 x 128 reported beside it
   composite      the 1M-node composite trilinear Euler-Cromer bar (three
                  bond types, three regimes): step time, bond-updates/s and
                  the HBM fraction with its algorithmic bytes
@@ -180,17 +183,37 @@ def uq_bench(nsamples, steps, tune_steps):
     ms_batched = b.run(steps, graph=True)
     live_end = b.integrator.live_bonds()
     del b
-    ms_seq = 0.0
-    for w in ws:
-        integ = batch.solo_integrator(w, lanes_per_node=best[0],
-                                      tile_nodes=best[1])
-        d, f = w.magnitudes(steps)
-        ms_seq += integ.timed_run(steps, d, f)
-        del integ
+
+    def sequential(shape, subset, nsteps):
+        """GPU ms of the samples run one after another, each in its own
+        simulation with back-to-back plain launches (pd_run)."""
+        tot = 0.0
+        for w in subset:
+            integ = batch.solo_integrator(w, lanes_per_node=shape[0],
+                                          tile_nodes=shape[1])
+            d, f = w.magnitudes(nsteps)
+            tot += integ.timed_run(nsteps, d, f)
+            del integ
+        return tot
+
+    # the sequential runs get their own launch shape: small meshes fill the
+    # GPU better with more, smaller workgroups than the batch does
+    default = (4, 128)
+    seq_cands = [default, (8, 64), (4, 64), (2, 128), (2, 256)]
+    spread = ws[::max(1, len(ws) // 16)]
+    seq_table = [(c, sequential(c, spread, 20)) for c in seq_cands]
+    seq_best = min(seq_table, key=lambda e: e[1])[0]
+    log("[bench] uq sequential shapes (16 samples, 20 steps): "
+        + " ".join(f"{c}:{ms:.1f}ms" for c, ms in seq_table))
+    ms_seq = sequential(seq_best, ws, steps)
+    ms_seq_default = (ms_seq if seq_best == default
+                      else sequential(default, ws, steps))
     ss_b = len(ws) * steps / (ms_batched / 1e3)
     ss_s = len(ws) * steps / (ms_seq / 1e3)
+    ss_d = len(ws) * steps / (ms_seq_default / 1e3)
     log(f"[bench] uq: batched {ms_batched:.1f} ms, sequential {ms_seq:.1f} "
-        f"ms for {steps} steps")
+        f"ms at {seq_best} ({ms_seq_default:.1f} ms at {default}) for "
+        f"{steps} steps")
     return {
         "workload": f"uq_{len(ws)}_samples_{nodes}_nodes",
         "samples": len(ws), "nodes": nodes, "steps": steps,
@@ -199,6 +222,11 @@ def uq_bench(nsamples, steps, tune_steps):
         "sequential_sample_steps_per_s": ss_s,
         "speedup": ss_b / ss_s,
         "batched_ms": ms_batched, "sequential_ms": ms_seq,
+        "sequential_launch_shape": {"lanes_per_node": seq_best[0],
+                                    "tile_nodes": seq_best[1]},
+        "sequential_shapes": [[list(c), ms] for c, ms in seq_table],
+        "sequential_default_shape_sample_steps_per_s": ss_d,
+        "speedup_vs_default_shape": ss_b / ss_d,
         "live_bonds_start": int(sum(w.n_neigh.sum() for w in ws)),
         "live_bonds_end": live_end,
         "launch_shape": {"lanes_per_node": best[0], "tile_nodes": best[1]},
