@@ -362,7 +362,9 @@ __device__ void compact_tile(const StepArgs &a, const TileInfo &ti,
     rank = __shfl(rank, 0, LPN);  // the node's first lane holds its rank
     const int K = a.K;
     const unsigned nb = compact_row_bytes(K);
-    const int G = (int)((bytes - 64u) / nb);  // >= 1: checked by the host
+    const int G = (int)((bytes - 64u) / nb);
+    if (G < 1)
+        return;  // rows too wide: rejected by the host before any launch
     const long long c0 = ti.chunk_off + l;
     const uint32_t m = mine ? (uint32_t)a.tnode[i] : 0u;
     unsigned short *R = a.rslot + (size_t)m * K;
@@ -1138,6 +1140,13 @@ __global__ void k_live(int n, const int *cnt, const long long *chunk0,
 // fold windows (12 doubles per node, tiles of up to EXACT_NODES nodes).
 constexpr int EXACT_NODES = 128;
 constexpr int EXACT_CAP = (int)((LDS_BYTES - EXACT_NODES * 96u) / 64u);
+
+// Rows of K slots fit compact_tile's groups in the exact step's fold
+// windows (its 64 bytes of counts, then at least one row).
+__host__ __device__ constexpr bool exact_rows_fit(int K)
+{
+    return compact_row_bytes(K) + 64u <= EXACT_NODES * 96u;
+}
 
 // The exact bond evaluation: the term's components and the stretch.
 struct ExactTerm {
@@ -2066,6 +2075,9 @@ int launch_step(pd_sim *s, double dmag, double fmag)
     if (s->exact && (s->lpn != 4 || s->tile_nodes > EXACT_NODES))
         return fail(PD_E_STATE, "the exact-order step runs 4 lanes per node "
                                 "and tiles of at most 128 nodes");
+    if (s->exact && !exact_rows_fit(s->K))
+        return fail(PD_E_LIMIT, "the exact-order step takes at most 4074 "
+                                "neighbours per node");
     const StepArgs a = make_args(s);
     hipError_t e;
     if (s->exact) {
@@ -3091,8 +3103,8 @@ int pd_set_exact_order(pd_sim *s, int on)
     if (on && (s->layout != PD_LAYOUT_COMPACT || s->integrator != PD_EULER))
         return fail(PD_E_ARG, "the exact-order step is the Cython path's: "
                               "PD_LAYOUT_COMPACT and PD_EULER only");
-    if (on && compact_row_bytes(s->K) + 16u > EXACT_NODES * 96u)
-        return fail(PD_E_LIMIT, "the exact-order step takes at most 4090 "
+    if (on && !exact_rows_fit(s->K))
+        return fail(PD_E_LIMIT, "the exact-order step takes at most 4074 "
                                 "neighbours per node");
     const bool was = s->exact;
     s->exact = on != 0;

@@ -158,6 +158,29 @@ def test_switching_and_rejection():
         _lib.check(_lib.lib().pd_set_exact_order(slot._sim, 1))
 
 
+def test_wide_rows_limit_the_exact_order():
+    """The exact step compacts rows in its fold windows (12 KiB per tile):
+    rows wider than 4074 slots are rejected with PD_E_LIMIT, while the fast
+    order (compacted in the halo's 80 KiB) steps them and agrees with the
+    oracle."""
+    w = configs.bar_euler(layout="compact", dt=5e-15, pull=1e-5)
+    wide = np.zeros((w.nnodes, 4075), dtype=w.nlist.dtype)
+    wide[:, :w.max_neighbours] = w.nlist
+    w.nlist = wide
+    with pytest.raises(_lib.PDError, match="4074"):
+        _pair(w)
+    integ, ref = _pair(w, exact=False)
+    d, f = w.magnitudes(20)
+    for s in range(20):
+        integ(d[s], f[s])
+        ref(d[s], f[s])
+    u, _, _, _, _, dmg, nl, nn = integ.write(
+        None, None, None, None, None, None, None, None)
+    np.testing.assert_array_equal(nl, ref.nlist)
+    np.testing.assert_array_equal(nn, ref.n_neigh)
+    assert pc.rel_l2(u, ref.u) < 1e-10
+
+
 def test_model_simulate_exact_order_bitwise(golden, tmp_path):
     """The whole model path of the reference's regression test
     (test_regression.py:36-79): Model with the Cython-layout Euler in

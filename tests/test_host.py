@@ -288,10 +288,10 @@ def test_build_rejects_bad_arguments_before_touching_a_device():
     bct = np.zeros((n, 3), np.int32)
     bcv = np.zeros((n, 3))
 
-    def build(family, kk=k, integrator=0, dens=None):
+    def build(family, kk=k, integrator=0, dens=None, layout=1):
         sim = ctypes.c_void_p()
         return _lib.lib().pd_build(
-            ctypes.byref(sim), 0, integrator, 1, 1e-3, 0.0, n, kk,
+            ctypes.byref(sim), 0, integrator, layout, 1e-3, 0.0, n, kk,
             _lib.dptr(coords), _lib.dptr(vol), _lib.iptr(family),
             _lib.iptr(bct), _lib.dptr(bcv), _lib.iptr(bct), _lib.dptr(bcv),
             None, None, _lib.dptr(dens))
@@ -304,3 +304,7 @@ def test_build_rejects_bad_arguments_before_touching_a_device():
     assert build(ok, dens=np.ones((n, 3))) == _lib.PD_E_ARG
     assert b"densities" in _lib.lib().pd_last_error()
     assert build(ok, integrator=2) == _lib.PD_E_ARG
+    # the compact layout's rows are compacted in LDS: at most 27285 slots
+    assert build(ok, kk=27286, layout=0) == _lib.PD_E_LIMIT
+    assert b"27285" in _lib.lib().pd_last_error()
+    assert build(ok, kk=27285, layout=0) != _lib.PD_E_LIMIT
