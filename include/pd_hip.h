@@ -81,7 +81,9 @@ int pd_device_name(int device, char *name, int name_len, char *arch,
  *   bond_types (N,K) i32 or NULL (bond_force3/4),
  *   densities (N,3) f64: required for EULER_CROMER / VELOCITY_VERLET and
  *   rejected for EULER (integrators.py:415-423, 511-519, 610-620).
- * max_neighbours is K, the width of every (N,K) array. */
+ * max_neighbours is K, the width of every (N,K) array: at most 65535, and
+ * at most 27285 for PD_LAYOUT_COMPACT, whose rows are compacted in LDS
+ * (PD_E_LIMIT beyond). */
 int pd_build(pd_sim **sim, int device, int integrator, int layout, double dt,
              double damping, int nnodes, int max_neighbours,
              const double *coords, const double *volume, const int *family,
@@ -132,11 +134,13 @@ int pd_set_break_mode(pd_sim *sim, int tension_only);
  * then the force boundary condition -- with the reference's operations in
  * IEEE double without contraction, so that force and u
  * (peridynamics.pyx:159-185) equal that arithmetic bitwise, not only to
- * rounding.  A verification mode: the force runs one thread per node with
- * global reads, several times slower than the default step.  Assumes
- * symmetric neighbour lists, as the model builds them.  0 (default) turns
- * it off.  No counterpart in the reference (it replaces the whole of
- * Euler.__call__, integrators.py:330-349). */
+ * rounding.  The step runs on the default step's tiles and LDS halo, four
+ * lanes per node folding consecutive list positions, at about twice the
+ * default step's time; integrators.Euler turns it on, the library's default
+ * is 0 (off).  Rows of at most 4074 slots (PD_E_LIMIT beyond).  Assumes
+ * symmetric neighbour lists without repeats, as the model builds them.  No
+ * counterpart in the reference (it replaces the whole of Euler.__call__,
+ * integrators.py:330-349, for the Cython path). */
 int pd_set_exact_order(pd_sim *sim, int on);
 
 /* Launch shape of the step kernel: lanes per node (1, 2, 4 or 8) and the
